@@ -1,0 +1,150 @@
+/*
+ * TileStream C-ABI: the drop-in boundary of this library.
+ *
+ * Flat `extern "C"` entry points over the C++ API (tilestream/hip.h and
+ * friends) with plain pointers and sizes; any FFI (ctypes, cgo, JNI, N-API) can
+ * bind them.  Every function returns a tilestream Status value (0 = success,
+ * numbering of reference include/tilestream/types.h:7-19, 9 = HIP runtime
+ * failure) unless stated otherwise.  Record types have exactly the layout of
+ * their C++ counterparts.  Streams are `void *` (hipStream_t).
+ *
+ * Reference interface each entry replaces:
+ *   ts_image_span / ts_compress_bound / ts_workspace_size
+ *                          <- reference include/tilestream/sizing.h:26-30
+ *   ts_parse_metadata      <- reference include/tilestream/api.h:42-43
+ *   ts_cpu_compress / ts_cpu_decompress <- reference include/tilestream/cpu.h:9-12
+ *   ts_prepare_batch       <- reference include/tilestream/cuda.h:176-177
+ *   ts_plan_workspace_spec <- reference include/tilestream/cuda.h:156
+ *   ts_compress_async, ts_decompress_async, ts_decode_roi_async,
+ *   ts_calibrate_async, ts_decode_calibrate_async
+ *                          <- reference include/tilestream/cuda.h:178-187
+ *   ts_buffer_allocate / ts_buffer_release <- reference include/tilestream/cuda.h:113-140
+ *   ts_synchronize         <- reference include/tilestream/cuda.h:145
+ * Extensions without a reference counterpart (batched persistent decode,
+ * autotuning, serving) are declared in the second half.
+ */
+#ifndef TILESTREAM_C_H
+#define TILESTREAM_C_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct { uint32_t type, width, height; } ts_image_spec;          /* PixelType 1/2 */
+typedef struct { uint16_t width, height; } ts_tile_shape;
+typedef struct { uint32_t x, y, width, height; } ts_roi;
+typedef struct { const void *data; uint64_t size; } ts_const_bytes;
+typedef struct { void *data; uint64_t size; } ts_mutable_bytes;
+typedef struct { const void *data; ts_image_spec image; uint64_t pitch_bytes, capacity_bytes; } ts_const_image_view;
+typedef struct { void *data; ts_image_spec image; uint64_t pitch_bytes, capacity_bytes; } ts_image_view;
+typedef struct { uint32_t status, reserved; uint64_t bytes_written; } ts_device_result;
+typedef struct { uint64_t image_count, tile_count; ts_tile_shape tile; uint64_t scan_temp_bytes; } ts_workspace_spec;
+typedef struct { uint64_t file_bytes, tile_count, metadata_bytes; } ts_parse_limits;
+
+/* Host metadata of a parsed archive (tilestream::ImageMetadata), opaque. */
+typedef struct ts_metadata ts_metadata;
+typedef struct {
+    ts_image_spec image;
+    ts_tile_shape tile;
+    uint64_t file_bytes, tile_count, segment_count;
+} ts_metadata_summary;
+typedef struct { uint32_t mode; uint16_t width, height; uint64_t offset, length, first_segment; } ts_tile_metadata;
+typedef struct { uint32_t seed, bit_width; uint64_t payload_offset, payload_bytes; } ts_segment_metadata;
+
+typedef struct { uint32_t kind; float scalar; ts_const_image_view map; } ts_parameter; /* kind 0 scalar, 1 map */
+typedef struct { ts_parameter offset, gain; } ts_calibration;
+typedef struct { uint64_t tile_index, input_offset; } ts_tile_mapping;
+typedef struct {
+    ts_const_bytes bytes;
+    const ts_metadata *metadata;
+    const ts_tile_mapping *mappings;
+    uint64_t mapping_count;
+    const ts_device_result *encoder_result;
+    ts_image_spec trusted_image;
+    ts_tile_shape trusted_tile;
+} ts_compressed_input;
+typedef struct {
+    ts_const_image_view input;
+    ts_image_view output;
+    ts_mutable_bytes compressed_output;
+    ts_compressed_input compressed_input;
+    ts_roi roi;
+    ts_calibration calibration;
+} ts_batch_item;
+/* operation: 0 compress, 1 decompress, 2 decode_roi, 3 calibrate, 4 decode_calibrate */
+typedef struct { uint32_t operation; const ts_batch_item *items; uint64_t count; ts_tile_shape tile; } ts_batch_request;
+
+typedef struct ts_plan ts_plan;
+
+/* ---- sizing, parser, host codec ---------------------------------------- */
+uint32_t ts_image_span(const ts_image_spec *image, uint64_t pitch_bytes, uint64_t *bytes);
+uint32_t ts_compress_bound(const ts_image_spec *image, ts_tile_shape tile, uint64_t *bytes);
+uint32_t ts_workspace_size(const ts_workspace_spec *spec, uint64_t *bytes);
+/* On success *out owns a new metadata object (free with ts_metadata_free). */
+uint32_t ts_parse_metadata(ts_const_bytes source, const ts_parse_limits *limits, ts_metadata **out);
+void ts_metadata_free(ts_metadata *metadata);
+void ts_metadata_get_summary(const ts_metadata *metadata, ts_metadata_summary *out);
+/* Copies / overwrites tile and segment records (tests fabricate metadata). */
+uint32_t ts_metadata_get_tile(const ts_metadata *metadata, uint64_t index, ts_tile_metadata *out);
+uint32_t ts_metadata_set_tile(ts_metadata *metadata, uint64_t index, const ts_tile_metadata *in);
+uint32_t ts_metadata_get_segment(const ts_metadata *metadata, uint64_t index, ts_segment_metadata *out);
+uint32_t ts_metadata_set_segment(ts_metadata *metadata, uint64_t index, const ts_segment_metadata *in);
+uint32_t ts_cpu_compress(ts_const_image_view input, ts_tile_shape tile, ts_mutable_bytes output,
+                         uint64_t *bytes_written);
+uint32_t ts_cpu_decompress(ts_const_bytes input, ts_image_view output, const ts_parse_limits *limits);
+
+/* ---- GPU plans and execution ------------------------------------------- */
+/* On success *plan is replaced (the previous plan, if any, is destroyed). */
+uint32_t ts_prepare_batch(const ts_batch_request *request, ts_plan **plan, void *stream);
+void ts_plan_destroy(ts_plan *plan);
+void ts_plan_workspace_spec(const ts_plan *plan, ts_workspace_spec *out);
+uint32_t ts_compress_async(const ts_plan *plan, ts_mutable_bytes workspace, ts_device_result *results,
+                           uint64_t result_count, void *stream);
+uint32_t ts_decompress_async(const ts_plan *plan, ts_mutable_bytes workspace, ts_device_result *results,
+                             uint64_t result_count, void *stream);
+uint32_t ts_decode_roi_async(const ts_plan *plan, ts_mutable_bytes workspace, ts_device_result *results,
+                             uint64_t result_count, void *stream);
+uint32_t ts_calibrate_async(const ts_plan *plan, ts_mutable_bytes workspace, ts_device_result *results,
+                            uint64_t result_count, void *stream);
+uint32_t ts_decode_calibrate_async(const ts_plan *plan, ts_mutable_bytes workspace, ts_device_result *results,
+                                   uint64_t result_count, void *stream);
+/* DeviceBuffer::allocate / release: stream-ordered, 256-byte aligned. */
+uint32_t ts_buffer_allocate(uint64_t bytes, void *stream, void **data);
+uint32_t ts_buffer_release(void *data, void *stream);
+uint32_t ts_synchronize(void *stream);
+
+/* ---- extension: launch shape of the decode kernel ---------------------- */
+/* group: columns per lane (1,2,4,8,16); block: 64..512; grid: 0 = one workgroup per tile. */
+uint32_t ts_plan_set_decode_shape(ts_plan *plan, uint32_t group, uint32_t block, uint32_t grid);
+
+/* ---- HIP runtime helpers for hosts without their own HIP binding --------
+ * These return hipError_t values (0 = hipSuccess). */
+int ts_hip_malloc(uint64_t bytes, void **data);
+int ts_hip_free(void *data);
+int ts_hip_host_malloc(uint64_t bytes, void **data); /* pinned, mapped */
+int ts_hip_host_free(void *data);
+int ts_hip_memcpy(void *dst, const void *src, uint64_t bytes);  /* any direction, synchronous */
+int ts_hip_memcpy_async(void *dst, const void *src, uint64_t bytes, void *stream);
+int ts_hip_memset(void *dst, int value, uint64_t bytes);
+int ts_hip_stream_create(void **stream);
+int ts_hip_stream_destroy(void *stream);
+int ts_hip_stream_synchronize(void *stream);
+int ts_hip_device_synchronize(void);
+int ts_hip_get_last_error(void);
+int ts_hip_peek_last_error(void);
+int ts_hip_device_count(int *count);
+int ts_hip_event_create(void **event);
+int ts_hip_event_destroy(void *event);
+int ts_hip_event_record(void *event, void *stream);
+int ts_hip_event_synchronize(void *event);
+int ts_hip_event_elapsed_ms(void *start, void *stop, float *ms);
+/* Sizes of the record types, for binding checks: name -> sizeof, 0 if unknown. */
+uint64_t ts_abi_sizeof(const char *name);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,164 @@
+/*
+ * TileStream kernel layer: the C-ABI between the host planner (C++) and the
+ * hand-written HIP kernels for gfx950.  Plain POD descriptor tables, device
+ * pointers, sizes and a hipStream_t passed as `void *`; every launcher returns
+ * a hipError_t value (0 = hipSuccess) and never allocates, synchronizes or
+ * loads code.
+ *
+ * These launchers replace the reference's kernel launches:
+ *   tsk_results_begin / tsk_results_complete  <- reference src/cuda/decoder.cu:307-326
+ *   tsk_decode                                 <- reference src/cuda/decoder.cu:176-305, 328-339
+ *   tsk_encode_analyze                         <- reference src/cuda/encoder.cu:62-138
+ *   tsk_encode_scan                            <- reference src/cuda/encoder.cu:284-288 (CUB scan)
+ *   tsk_encode_pack                            <- reference src/cuda/encoder.cu:175-238
+ *   tsk_encode_finish                          <- reference src/cuda/encoder.cu:240-280
+ *   tsk_calibrate / tsk_calibrate_report       <- reference src/cuda/calibrate.cu:13-62
+ *   tsk_preload                                <- reference src/cuda/decoder.cu:341-349,
+ *                                                 encoder.cu:329-349, calibrate.cu:64-71
+ */
+#ifndef TILESTREAM_KERNELS_H
+#define TILESTREAM_KERNELS_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TSK_NO_ROWS 0xffffffffu
+
+/* Pixel kinds of a decode image / task. */
+enum { TSK_UINT16 = 1, TSK_FLOAT32 = 2 };
+
+/* Decode kernel variants: fixed pixel type and calibration, or mixed. */
+enum {
+    TSK_KIND_U16 = 0,
+    TSK_KIND_F32 = 1,
+    TSK_KIND_U16_CAL = 2,
+    TSK_KIND_F32_CAL = 3,
+    TSK_KIND_MIXED = 4
+};
+
+/* Per-output descriptor of a decode (or fused decode + calibrate) job. */
+typedef struct {
+    const uint8_t *src;          /* compressed bytes (full file, or staged payloads) */
+    uint8_t *out;                /* output pixel at the region's top-left corner */
+    const void *producer;        /* trusted path: the encoder's DeviceResult, else NULL */
+    const uint8_t *offset_map;   /* FP32 maps in region coordinates, NULL = scalar */
+    const uint8_t *gain_map;
+    uint64_t src_size;           /* readable bytes at src */
+    uint64_t out_pitch;
+    uint64_t offset_pitch;
+    uint64_t gain_pitch;
+    uint32_t width, height;      /* archive image */
+    uint32_t roi_x, roi_y, roi_w, roi_h;
+    uint32_t tile_columns;       /* ceil(width / tile_w) */
+    uint16_t tile_w, tile_h;
+    uint8_t type;                /* TSK_UINT16 / TSK_FLOAT32 */
+    uint8_t calibrated;          /* output FP32 (x - offset) * gain */
+    uint16_t reserved;
+    float offset, gain;          /* scalar parameters */
+} tsk_decode_image;
+
+/* One tile to decode.  On the trusted path mode/offset/words are read from the
+ * device-resident index instead. */
+typedef struct {
+    uint64_t offset;             /* payload byte offset within image.src */
+    uint64_t tile_index;         /* tile number in the archive */
+    uint32_t image;              /* index into the image table */
+    uint32_t rows;               /* first entry of the row table, TSK_NO_ROWS if none */
+    uint32_t words;              /* payload length / 4 */
+    uint32_t mode;               /* 0 RAW, 1 CONSTANT, 2 PREDICT_BITPACK */
+} tsk_decode_task;
+
+typedef struct {
+    const tsk_decode_image *images;
+    const tsk_decode_task *tasks;
+    const uint16_t *rows;        /* word offset of each predictive row in its payload */
+    void *results;               /* DeviceResult[image_count] */
+    uint32_t *queue;             /* work counter for the dynamic queue, NULL = static stride */
+    uint64_t task_count;
+    uint32_t image_count;
+    uint32_t kind;               /* TSK_KIND_* */
+    uint32_t group;              /* columns per lane: 1, 2, 4, 8 or 16 */
+    uint32_t block;              /* threads per workgroup: 64, 128, 256 or 512 */
+    uint32_t grid;               /* workgroups; tiles are strided or drawn from the queue */
+    uint32_t chunk;              /* tiles taken per queue draw */
+} tsk_decode_args;
+
+/* Per-image descriptor of an encode job. */
+typedef struct {
+    const uint8_t *in;
+    uint8_t *out;                /* .tsc file, capacity >= compress_bound */
+    uint64_t in_pitch;
+    uint64_t tile_count;
+    uint32_t width, height;
+    uint32_t tile_columns;
+    uint16_t tile_w, tile_h;
+    uint32_t type;
+} tsk_encode_image;
+
+/* The part of one image that falls into one encode window. */
+typedef struct {
+    uint32_t image;
+    uint32_t first;              /* first window tile of this image */
+    uint32_t count;
+    uint32_t flags;              /* 1: continues an earlier window, 2: finishes the image */
+    uint64_t start;              /* image tile number of the first window tile */
+} tsk_window_entry;
+
+typedef struct {
+    const tsk_encode_image *images;
+    const tsk_window_entry *entries;
+    void *results;               /* DeviceResult per image */
+    uint8_t *rows;               /* workspace: per tile, tile_h row widths (1 byte each) */
+    uint32_t *tiles;             /* workspace: per tile {mode, length} */
+    uint64_t *offsets;           /* workspace: window prefix of payload lengths, tile_count + 1 */
+    uint64_t *carry;             /* workspace: 2 slots, output cursor of a continued image */
+    uint32_t entry_count;
+    uint32_t tile_count;         /* tiles in this window */
+    uint32_t parity;             /* window number & 1 */
+    uint32_t max_rows;           /* row slots per tile in the workspace */
+    uint32_t type;               /* TSK_UINT16 / TSK_FLOAT32 */
+} tsk_encode_args;
+
+/* Per-image descriptor of standalone calibration. */
+typedef struct {
+    const uint8_t *in;
+    uint8_t *out;
+    const uint8_t *offset_map;
+    const uint8_t *gain_map;
+    uint64_t in_pitch, out_pitch, offset_pitch, gain_pitch;
+    uint32_t width, height;
+    float offset, gain;
+} tsk_calibrate_image;
+
+typedef struct {
+    const tsk_calibrate_image *images;
+    const uint64_t *first_rows;  /* first row of each image in the concatenated rows */
+    void *results;
+    uint64_t total_rows;
+    uint32_t image_count;
+    uint32_t type;
+} tsk_calibrate_args;
+
+int tsk_results_begin(void *results, uint32_t count, void *stream);
+int tsk_results_complete(const tsk_decode_image *images, uint32_t count, void *results,
+                         void *stream);
+int tsk_decode(const tsk_decode_args *args, void *stream);
+int tsk_encode_analyze(const tsk_encode_args *args, void *stream);
+int tsk_encode_scan(const tsk_encode_args *args, void *stream);
+int tsk_encode_pack(const tsk_encode_args *args, void *stream);
+int tsk_encode_finish(const tsk_encode_args *args, void *stream);
+int tsk_calibrate(const tsk_calibrate_args *args, void *stream);
+/* Loads every kernel of an operation so that no execution loads code.
+ * op: 0 compress, 1 decode, 2 calibrate, 3 fused decode. */
+int tsk_preload(uint32_t op);
+/* Zero-fills `words` 32-bit words (a kernel, safe under graph capture). */
+int tsk_fill_u32(uint32_t *data, uint64_t words, uint32_t value, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,220 @@
+"""ctypes binding of the TileStream C-ABI (include/tilestream_c.h).
+
+Loads tilestream_amd/_native/libtilestream_amd.so built in-tree by
+`python -m tilestream_amd.build`.  There is no fallback: if the library is
+missing, every call raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "_native", "libtilestream_amd.so")
+GEN_PATH = os.path.join(HERE, "_native", "libtsgen.so")
+
+
+class NativeLibraryMissing(RuntimeError):
+    pass
+
+
+# ---- record types (layout-identical to tilestream_c.h) ------------------------
+
+class ImageSpec(C.Structure):
+    _fields_ = [("type", C.c_uint32), ("width", C.c_uint32), ("height", C.c_uint32)]
+
+
+class TileShape(C.Structure):
+    _fields_ = [("width", C.c_uint16), ("height", C.c_uint16)]
+
+    def __init__(self, width=128, height=32):
+        super().__init__(width, height)
+
+
+class Roi(C.Structure):
+    _fields_ = [("x", C.c_uint32), ("y", C.c_uint32), ("width", C.c_uint32), ("height", C.c_uint32)]
+
+
+class ConstBytes(C.Structure):
+    _fields_ = [("data", C.c_void_p), ("size", C.c_uint64)]
+
+
+class MutableBytes(C.Structure):
+    _fields_ = [("data", C.c_void_p), ("size", C.c_uint64)]
+
+
+class ConstImageView(C.Structure):
+    _fields_ = [("data", C.c_void_p), ("image", ImageSpec), ("pitch_bytes", C.c_uint64),
+                ("capacity_bytes", C.c_uint64)]
+
+
+class ImageView(C.Structure):
+    _fields_ = [("data", C.c_void_p), ("image", ImageSpec), ("pitch_bytes", C.c_uint64),
+                ("capacity_bytes", C.c_uint64)]
+
+
+class DeviceResult(C.Structure):
+    _fields_ = [("status", C.c_uint32), ("reserved", C.c_uint32), ("bytes_written", C.c_uint64)]
+
+
+class WorkspaceSpec(C.Structure):
+    _fields_ = [("image_count", C.c_uint64), ("tile_count", C.c_uint64), ("tile", TileShape),
+                ("scan_temp_bytes", C.c_uint64)]
+
+
+class ParseLimits(C.Structure):
+    _fields_ = [("file_bytes", C.c_uint64), ("tile_count", C.c_uint64), ("metadata_bytes", C.c_uint64)]
+
+
+class MetadataSummary(C.Structure):
+    _fields_ = [("image", ImageSpec), ("tile", TileShape), ("file_bytes", C.c_uint64),
+                ("tile_count", C.c_uint64), ("segment_count", C.c_uint64)]
+
+
+class TileMetadata(C.Structure):
+    _fields_ = [("mode", C.c_uint32), ("width", C.c_uint16), ("height", C.c_uint16),
+                ("offset", C.c_uint64), ("length", C.c_uint64), ("first_segment", C.c_uint64)]
+
+
+class SegmentMetadata(C.Structure):
+    _fields_ = [("seed", C.c_uint32), ("bit_width", C.c_uint32), ("payload_offset", C.c_uint64),
+                ("payload_bytes", C.c_uint64)]
+
+
+class Parameter(C.Structure):
+    _fields_ = [("kind", C.c_uint32), ("scalar", C.c_float), ("map", ConstImageView)]
+
+
+class Calibration(C.Structure):
+    _fields_ = [("offset", Parameter), ("gain", Parameter)]
+
+    def __init__(self, *args, **kw):
+        super().__init__(*args, **kw)
+        if not args and "gain" not in kw:
+            self.gain.scalar = 1.0
+
+
+class TileMapping(C.Structure):
+    _fields_ = [("tile_index", C.c_uint64), ("input_offset", C.c_uint64)]
+
+
+class CompressedInput(C.Structure):
+    _fields_ = [("bytes", ConstBytes), ("metadata", C.c_void_p), ("mappings", C.POINTER(TileMapping)),
+                ("mapping_count", C.c_uint64), ("encoder_result", C.c_void_p),
+                ("trusted_image", ImageSpec), ("trusted_tile", TileShape)]
+
+
+class BatchItem(C.Structure):
+    _fields_ = [("input", ConstImageView), ("output", ImageView), ("compressed_output", MutableBytes),
+                ("compressed_input", CompressedInput), ("roi", Roi), ("calibration", Calibration)]
+
+    def __init__(self, *args, **kw):
+        super().__init__(*args, **kw)
+        if "calibration" not in kw:
+            self.calibration.gain.scalar = 1.0
+
+
+class BatchRequest(C.Structure):
+    _fields_ = [("operation", C.c_uint32), ("items", C.POINTER(BatchItem)), ("count", C.c_uint64),
+                ("tile", TileShape)]
+
+
+RECORDS = {
+    "ts_image_spec": ImageSpec, "ts_tile_shape": TileShape, "ts_roi": Roi,
+    "ts_const_bytes": ConstBytes, "ts_mutable_bytes": MutableBytes,
+    "ts_const_image_view": ConstImageView, "ts_image_view": ImageView,
+    "ts_device_result": DeviceResult, "ts_workspace_spec": WorkspaceSpec,
+    "ts_parse_limits": ParseLimits, "ts_metadata_summary": MetadataSummary,
+    "ts_tile_metadata": TileMetadata, "ts_segment_metadata": SegmentMetadata,
+    "ts_parameter": Parameter, "ts_calibration": Calibration, "ts_tile_mapping": TileMapping,
+    "ts_compressed_input": CompressedInput, "ts_batch_item": BatchItem,
+    "ts_batch_request": BatchRequest,
+}
+
+_u32, _u64, _vp, _int = C.c_uint32, C.c_uint64, C.c_void_p, C.c_int
+_P = C.POINTER
+
+# name -> (restype, argtypes)
+PROTOTYPES = {
+    "ts_image_span": (_u32, [_P(ImageSpec), _u64, _P(_u64)]),
+    "ts_compress_bound": (_u32, [_P(ImageSpec), TileShape, _P(_u64)]),
+    "ts_workspace_size": (_u32, [_P(WorkspaceSpec), _P(_u64)]),
+    "ts_parse_metadata": (_u32, [ConstBytes, _P(ParseLimits), _P(_vp)]),
+    "ts_metadata_free": (None, [_vp]),
+    "ts_metadata_get_summary": (None, [_vp, _P(MetadataSummary)]),
+    "ts_metadata_get_tile": (_u32, [_vp, _u64, _P(TileMetadata)]),
+    "ts_metadata_set_tile": (_u32, [_vp, _u64, _P(TileMetadata)]),
+    "ts_metadata_get_segment": (_u32, [_vp, _u64, _P(SegmentMetadata)]),
+    "ts_metadata_set_segment": (_u32, [_vp, _u64, _P(SegmentMetadata)]),
+    "ts_cpu_compress": (_u32, [ConstImageView, TileShape, MutableBytes, _P(_u64)]),
+    "ts_cpu_decompress": (_u32, [ConstBytes, ImageView, _P(ParseLimits)]),
+    "ts_prepare_batch": (_u32, [_P(BatchRequest), _P(_vp), _vp]),
+    "ts_plan_destroy": (None, [_vp]),
+    "ts_plan_workspace_spec": (None, [_vp, _P(WorkspaceSpec)]),
+    "ts_compress_async": (_u32, [_vp, MutableBytes, _vp, _u64, _vp]),
+    "ts_decompress_async": (_u32, [_vp, MutableBytes, _vp, _u64, _vp]),
+    "ts_decode_roi_async": (_u32, [_vp, MutableBytes, _vp, _u64, _vp]),
+    "ts_calibrate_async": (_u32, [_vp, MutableBytes, _vp, _u64, _vp]),
+    "ts_decode_calibrate_async": (_u32, [_vp, MutableBytes, _vp, _u64, _vp]),
+    "ts_buffer_allocate": (_u32, [_u64, _vp, _P(_vp)]),
+    "ts_buffer_release": (_u32, [_vp, _vp]),
+    "ts_synchronize": (_u32, [_vp]),
+    "ts_plan_set_decode_shape": (_u32, [_vp, _u32, _u32, _u32]),
+    "ts_hip_malloc": (_int, [_u64, _P(_vp)]),
+    "ts_hip_free": (_int, [_vp]),
+    "ts_hip_host_malloc": (_int, [_u64, _P(_vp)]),
+    "ts_hip_host_free": (_int, [_vp]),
+    "ts_hip_memcpy": (_int, [_vp, _vp, _u64]),
+    "ts_hip_memcpy_async": (_int, [_vp, _vp, _u64, _vp]),
+    "ts_hip_memset": (_int, [_vp, _int, _u64]),
+    "ts_hip_stream_create": (_int, [_P(_vp)]),
+    "ts_hip_stream_destroy": (_int, [_vp]),
+    "ts_hip_stream_synchronize": (_int, [_vp]),
+    "ts_hip_device_synchronize": (_int, []),
+    "ts_hip_get_last_error": (_int, []),
+    "ts_hip_peek_last_error": (_int, []),
+    "ts_hip_device_count": (_int, [_P(_int)]),
+    "ts_hip_event_create": (_int, [_P(_vp)]),
+    "ts_hip_event_destroy": (_int, [_vp]),
+    "ts_hip_event_record": (_int, [_vp, _vp]),
+    "ts_hip_event_synchronize": (_int, [_vp]),
+    "ts_hip_event_elapsed_ms": (_int, [_vp, _vp, _P(C.c_float)]),
+    "ts_abi_sizeof": (_u64, [C.c_char_p]),
+}
+
+_lib = None
+_gen = None
+
+
+def lib():
+    """The loaded product library; raises if it was not built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise NativeLibraryMissing(
+                f"{LIB_PATH} is missing: build it with `python -m tilestream_amd.build`")
+        handle = C.CDLL(LIB_PATH)
+        for name, (res, args) in PROTOTYPES.items():
+            fn = getattr(handle, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = handle
+    return _lib
+
+
+def gen():
+    global _gen
+    if _gen is None:
+        if not os.path.exists(GEN_PATH):
+            raise NativeLibraryMissing(f"{GEN_PATH} is missing: run `python -m tilestream_amd.build`")
+        g = C.CDLL(GEN_PATH)
+        g.tsg_detector_u16.argtypes = [_u64, _u32, _u32, _vp, _u64, _int]
+        g.tsg_field_f32.argtypes = [_u64, _u32, _u32, _vp, _u64, _int]
+        g.tsg_dark_u16.argtypes = [_u64, _u32, _u32, _vp, _int]
+        g.tsg_gain_f32.argtypes = [_u64, _u32, _u32, _vp, _int]
+        g.tsg_bits.restype = _u64
+        g.tsg_bits.argtypes = [_u64, _u64, _u64]
+        for f in (g.tsg_detector_u16, g.tsg_field_f32, g.tsg_dark_u16, g.tsg_gain_f32):
+            f.restype = None
+        _gen = g
+    return _gen

@@ -1,0 +1,276 @@
+// The C-ABI of tilestream_c.h over the C++ API.  Record types are
+// layout-identical to their C++ counterparts (checked below), so arrays are
+// passed through without copying.
+#include "plan.h"
+
+#include <tilestream/cpu.h>
+#include <tilestream/hip.h>
+#include <tilestream_c.h>
+
+#include <cstddef>
+#include <cstring>
+#include <memory>
+#include <new>
+#include <string>
+
+using namespace tilestream;
+
+#define TS_SAME_LAYOUT(C_TYPE, CXX_TYPE) \
+    static_assert(sizeof(C_TYPE) == sizeof(CXX_TYPE) && alignof(C_TYPE) == alignof(CXX_TYPE), #C_TYPE)
+
+TS_SAME_LAYOUT(ts_image_spec, ImageSpec);
+TS_SAME_LAYOUT(ts_tile_shape, TileShape);
+TS_SAME_LAYOUT(ts_roi, Roi);
+TS_SAME_LAYOUT(ts_const_bytes, ConstBytes);
+TS_SAME_LAYOUT(ts_mutable_bytes, MutableBytes);
+TS_SAME_LAYOUT(ts_const_image_view, ConstImageView);
+TS_SAME_LAYOUT(ts_image_view, ImageView);
+TS_SAME_LAYOUT(ts_device_result, DeviceResult);
+TS_SAME_LAYOUT(ts_workspace_spec, WorkspaceSpec);
+TS_SAME_LAYOUT(ts_parse_limits, ParseLimits);
+TS_SAME_LAYOUT(ts_parameter, hip::Parameter);
+TS_SAME_LAYOUT(ts_calibration, hip::Calibration);
+TS_SAME_LAYOUT(ts_tile_mapping, hip::TileMapping);
+TS_SAME_LAYOUT(ts_compressed_input, hip::CompressedInput);
+TS_SAME_LAYOUT(ts_batch_item, hip::BatchItem);
+TS_SAME_LAYOUT(ts_batch_request, hip::BatchRequest);
+TS_SAME_LAYOUT(ts_tile_metadata, TileMetadata);
+TS_SAME_LAYOUT(ts_segment_metadata, SegmentMetadata);
+static_assert(offsetof(ts_batch_item, calibration) == offsetof(hip::BatchItem, calibration));
+static_assert(offsetof(ts_batch_item, roi) == offsetof(hip::BatchItem, roi));
+static_assert(offsetof(ts_compressed_input, trusted_tile) == offsetof(hip::CompressedInput, trusted_tile));
+static_assert(offsetof(ts_parameter, map) == offsetof(hip::Parameter, map));
+static_assert(offsetof(ts_batch_request, tile) == offsetof(hip::BatchRequest, tile));
+
+struct ts_plan {
+    std::unique_ptr<hip::BatchPlan> plan;
+};
+
+namespace {
+
+template <typename To, typename From> To &as(From &v) { return *reinterpret_cast<To *>(&v); }
+template <typename To, typename From> const To &as(const From &v) { return *reinterpret_cast<const To *>(&v); }
+
+ImageMetadata *meta(ts_metadata *m) { return reinterpret_cast<ImageMetadata *>(m); }
+const ImageMetadata *meta(const ts_metadata *m) { return reinterpret_cast<const ImageMetadata *>(m); }
+
+uint32_t code(Status s) { return static_cast<uint32_t>(s); }
+
+hipStream_t stream_of(void *s) { return static_cast<hipStream_t>(s); }
+
+using Exec = Status (*)(const hip::BatchPlan &, MutableBytes, hip::ResultBuffer, hipStream_t) noexcept;
+
+uint32_t exec(Exec fn, const ts_plan *plan, ts_mutable_bytes ws, ts_device_result *results, uint64_t n, void *s) {
+    if (plan == nullptr || plan->plan == nullptr) return code(Status::invalid_argument);
+    return code(fn(*plan->plan, as<MutableBytes>(ws), {reinterpret_cast<DeviceResult *>(results), n}, stream_of(s)));
+}
+
+} // namespace
+
+extern "C" {
+
+uint32_t ts_image_span(const ts_image_spec *image, uint64_t pitch, uint64_t *bytes) {
+    return code(image_span(as<ImageSpec>(*image), pitch, *bytes));
+}
+
+uint32_t ts_compress_bound(const ts_image_spec *image, ts_tile_shape tile, uint64_t *bytes) {
+    return code(compress_bound(as<ImageSpec>(*image), as<TileShape>(tile), *bytes));
+}
+
+uint32_t ts_workspace_size(const ts_workspace_spec *spec, uint64_t *bytes) {
+    return code(workspace_size(as<WorkspaceSpec>(*spec), *bytes));
+}
+
+uint32_t ts_parse_metadata(ts_const_bytes source, const ts_parse_limits *limits, ts_metadata **out) {
+    try {
+        auto parsed = std::make_unique<ImageMetadata>();
+        const ParseLimits lim = limits != nullptr ? as<ParseLimits>(*limits) : ParseLimits{};
+        const Status st = parse_metadata(as<ConstBytes>(source), lim, *parsed);
+        if (st == Status::success) *out = reinterpret_cast<ts_metadata *>(parsed.release());
+        return code(st);
+    } catch (const std::bad_alloc &) {
+        return code(Status::resource_limit);
+    }
+}
+
+void ts_metadata_free(ts_metadata *m) { delete meta(m); }
+
+void ts_metadata_get_summary(const ts_metadata *m, ts_metadata_summary *out) {
+    const ImageMetadata &x = *meta(m);
+    out->image = as<ts_image_spec>(x.image);
+    out->tile = as<ts_tile_shape>(x.tile);
+    out->file_bytes = x.file_bytes;
+    out->tile_count = x.tiles.size();
+    out->segment_count = x.segments.size();
+}
+
+uint32_t ts_metadata_get_tile(const ts_metadata *m, uint64_t i, ts_tile_metadata *out) {
+    if (i >= meta(m)->tiles.size()) return code(Status::invalid_argument);
+    std::memcpy(out, &meta(m)->tiles[i], sizeof *out);
+    return 0;
+}
+
+uint32_t ts_metadata_set_tile(ts_metadata *m, uint64_t i, const ts_tile_metadata *in) {
+    if (i >= meta(m)->tiles.size()) return code(Status::invalid_argument);
+    std::memcpy(&meta(m)->tiles[i], in, sizeof *in);
+    return 0;
+}
+
+uint32_t ts_metadata_get_segment(const ts_metadata *m, uint64_t i, ts_segment_metadata *out) {
+    if (i >= meta(m)->segments.size()) return code(Status::invalid_argument);
+    std::memcpy(out, &meta(m)->segments[i], sizeof *out);
+    return 0;
+}
+
+uint32_t ts_metadata_set_segment(ts_metadata *m, uint64_t i, const ts_segment_metadata *in) {
+    if (i >= meta(m)->segments.size()) return code(Status::invalid_argument);
+    std::memcpy(&meta(m)->segments[i], in, sizeof *in);
+    return 0;
+}
+
+uint32_t ts_cpu_compress(ts_const_image_view input, ts_tile_shape tile, ts_mutable_bytes output, uint64_t *written) {
+    return code(cpu::compress(as<ConstImageView>(input), as<TileShape>(tile), as<MutableBytes>(output), *written));
+}
+
+uint32_t ts_cpu_decompress(ts_const_bytes input, ts_image_view output, const ts_parse_limits *limits) {
+    const ParseLimits lim = limits != nullptr ? as<ParseLimits>(*limits) : ParseLimits{};
+    return code(cpu::decompress(as<ConstBytes>(input), as<ImageView>(output), lim));
+}
+
+uint32_t ts_prepare_batch(const ts_batch_request *request, ts_plan **plan, void *stream) {
+    if (request == nullptr || plan == nullptr) return code(Status::invalid_argument);
+    try {
+        std::unique_ptr<hip::BatchPlan> fresh;
+        const Status st = hip::prepare_batch(as<hip::BatchRequest>(*request), fresh, stream_of(stream));
+        if (st != Status::success) return code(st);
+        auto *handle = new ts_plan{std::move(fresh)};
+        delete *plan;
+        *plan = handle;
+        return 0;
+    } catch (const std::bad_alloc &) {
+        return code(Status::resource_limit);
+    }
+}
+
+void ts_plan_destroy(ts_plan *plan) { delete plan; }
+
+void ts_plan_workspace_spec(const ts_plan *plan, ts_workspace_spec *out) {
+    const WorkspaceSpec spec = plan != nullptr && plan->plan ? plan->plan->workspace_spec() : WorkspaceSpec{};
+    std::memcpy(out, &spec, sizeof spec);
+}
+
+uint32_t ts_compress_async(const ts_plan *p, ts_mutable_bytes ws, ts_device_result *r, uint64_t n, void *s) {
+    return exec(hip::compress_async, p, ws, r, n, s);
+}
+uint32_t ts_decompress_async(const ts_plan *p, ts_mutable_bytes ws, ts_device_result *r, uint64_t n, void *s) {
+    return exec(hip::decompress_async, p, ws, r, n, s);
+}
+uint32_t ts_decode_roi_async(const ts_plan *p, ts_mutable_bytes ws, ts_device_result *r, uint64_t n, void *s) {
+    return exec(hip::decode_roi_async, p, ws, r, n, s);
+}
+uint32_t ts_calibrate_async(const ts_plan *p, ts_mutable_bytes ws, ts_device_result *r, uint64_t n, void *s) {
+    return exec(hip::calibrate_async, p, ws, r, n, s);
+}
+uint32_t ts_decode_calibrate_async(const ts_plan *p, ts_mutable_bytes ws, ts_device_result *r, uint64_t n, void *s) {
+    return exec(hip::decode_calibrate_async, p, ws, r, n, s);
+}
+
+uint32_t ts_buffer_allocate(uint64_t bytes, void *stream, void **data) {
+    // DeviceBuffer::allocate rules, with ownership handed to the caller.
+    if (bytes > max_bytes) return code(Status::size_overflow);
+    if (bytes == 0) {
+        *data = nullptr;
+        return 0;
+    }
+    void *p = nullptr;
+    if (hipMallocAsync(&p, static_cast<std::size_t>(bytes), stream_of(stream)) != hipSuccess)
+        return code(hip::detail::hip_failure());
+    if (reinterpret_cast<std::uintptr_t>(p) % workspace_alignment != 0) {
+        static_cast<void>(hipFreeAsync(p, stream_of(stream)));
+        return code(Status::hip_error);
+    }
+    *data = p;
+    return 0;
+}
+
+uint32_t ts_buffer_release(void *data, void *stream) {
+    if (data == nullptr) return 0;
+    return code(hip::detail::hip_status(hipFreeAsync(data, stream_of(stream))));
+}
+
+uint32_t ts_synchronize(void *stream) { return code(hip::synchronize(stream_of(stream))); }
+
+uint32_t ts_plan_set_decode_shape(ts_plan *plan, uint32_t group, uint32_t block, uint32_t grid) {
+    if (plan == nullptr || plan->plan == nullptr) return code(Status::invalid_argument);
+    if ((group != 1 && group != 2 && group != 4 && group != 8 && group != 16) ||
+        (block != 64 && block != 128 && block != 256 && block != 512))
+        return code(Status::invalid_argument);
+    auto *state = const_cast<hip::BatchPlan::Impl *>(plan->plan->impl());
+    state->shape = {group, block, grid};
+    return 0;
+}
+
+int ts_hip_malloc(uint64_t bytes, void **data) { return hipMalloc(data, bytes); }
+int ts_hip_free(void *data) { return hipFree(data); }
+int ts_hip_host_malloc(uint64_t bytes, void **data) { return hipHostMalloc(data, bytes, hipHostMallocMapped); }
+int ts_hip_host_free(void *data) { return hipHostFree(data); }
+int ts_hip_memcpy(void *dst, const void *src, uint64_t bytes) { return hipMemcpy(dst, src, bytes, hipMemcpyDefault); }
+int ts_hip_memcpy_async(void *dst, const void *src, uint64_t bytes, void *stream) {
+    return hipMemcpyAsync(dst, src, bytes, hipMemcpyDefault, stream_of(stream));
+}
+int ts_hip_memset(void *dst, int value, uint64_t bytes) { return hipMemset(dst, value, bytes); }
+int ts_hip_stream_create(void **stream) {
+    hipStream_t s = nullptr;
+    const hipError_t e = hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
+    *stream = s;
+    return e;
+}
+int ts_hip_stream_destroy(void *stream) { return hipStreamDestroy(stream_of(stream)); }
+int ts_hip_stream_synchronize(void *stream) { return hipStreamSynchronize(stream_of(stream)); }
+int ts_hip_device_synchronize(void) { return hipDeviceSynchronize(); }
+int ts_hip_get_last_error(void) { return hipGetLastError(); }
+int ts_hip_peek_last_error(void) { return hipPeekAtLastError(); }
+int ts_hip_device_count(int *count) { return hipGetDeviceCount(count); }
+int ts_hip_event_create(void **event) {
+    hipEvent_t e = nullptr;
+    const hipError_t r = hipEventCreate(&e);
+    *event = e;
+    return r;
+}
+int ts_hip_event_destroy(void *event) { return hipEventDestroy(static_cast<hipEvent_t>(event)); }
+int ts_hip_event_record(void *event, void *stream) {
+    return hipEventRecord(static_cast<hipEvent_t>(event), stream_of(stream));
+}
+int ts_hip_event_synchronize(void *event) { return hipEventSynchronize(static_cast<hipEvent_t>(event)); }
+int ts_hip_event_elapsed_ms(void *start, void *stop, float *ms) {
+    return hipEventElapsedTime(ms, static_cast<hipEvent_t>(start), static_cast<hipEvent_t>(stop));
+}
+
+uint64_t ts_abi_sizeof(const char *name) {
+    const std::string n = name;
+#define TS_SIZE(T) \
+    if (n == #T) return sizeof(T);
+    TS_SIZE(ts_image_spec)
+    TS_SIZE(ts_tile_shape)
+    TS_SIZE(ts_roi)
+    TS_SIZE(ts_const_bytes)
+    TS_SIZE(ts_mutable_bytes)
+    TS_SIZE(ts_const_image_view)
+    TS_SIZE(ts_image_view)
+    TS_SIZE(ts_device_result)
+    TS_SIZE(ts_workspace_spec)
+    TS_SIZE(ts_parse_limits)
+    TS_SIZE(ts_metadata_summary)
+    TS_SIZE(ts_tile_metadata)
+    TS_SIZE(ts_segment_metadata)
+    TS_SIZE(ts_parameter)
+    TS_SIZE(ts_calibration)
+    TS_SIZE(ts_tile_mapping)
+    TS_SIZE(ts_compressed_input)
+    TS_SIZE(ts_batch_item)
+    TS_SIZE(ts_batch_request)
+#undef TS_SIZE
+    return 0;
+}
+
+} // extern "C"

@@ -1,0 +1,730 @@
+// Batch preparation: host validation, compact device tables, upload.
+//
+// Validation order and the Status of every failure follow the reference
+// planner (reference src/cuda/plan.cpp:128-465, 501-711) so that a caller sees
+// the same result for the same request.  The device tables are this library's:
+// one 32-byte task per tile and a 16-bit word offset per predictive row; the
+// row seeds and widths are read from the staged payload itself.
+#include "plan.h"
+
+#include <tilestream/format.h>
+
+#include <algorithm>
+#include <cstring>
+#include <new>
+#include <stdexcept>
+#include <utility>
+
+namespace tilestream::hip {
+namespace detail {
+
+using host::byte_range;
+
+bool RangeSet::assign(std::vector<Range> writable, std::vector<Range> readable) {
+    const auto empty = [](const Range &r) { return r.begin == r.end; };
+    const auto by_begin = [](const Range &a, const Range &b) { return a.begin < b.begin; };
+    writable.erase(std::remove_if(writable.begin(), writable.end(), empty), writable.end());
+    readable.erase(std::remove_if(readable.begin(), readable.end(), empty), readable.end());
+    std::sort(writable.begin(), writable.end(), by_begin);
+    std::sort(readable.begin(), readable.end(), by_begin);
+    for (std::size_t i = 1; i < writable.size(); ++i)
+        if (writable[i].begin < writable[i - 1].end) return false;
+    std::vector<std::uintptr_t> reach(readable.size());
+    std::uintptr_t far = 0;
+    for (std::size_t i = 0; i < readable.size(); ++i) reach[i] = far = std::max(far, readable[i].end);
+    writable_.clear();
+    readable_ = std::move(readable);
+    reach_ = std::move(reach);
+    for (const Range &r : writable)
+        if (!disjoint(r)) return false;
+    writable_ = std::move(writable);
+    return true;
+}
+
+bool RangeSet::disjoint(Range r) const noexcept {
+    if (r.begin == r.end) return true;
+    const auto starts_before = [](const Range &item, std::uintptr_t end) { return item.begin < end; };
+    // The last writable range starting before r.end is the only one that can reach into r.
+    const auto w = std::lower_bound(writable_.begin(), writable_.end(), r.end, starts_before);
+    if (w != writable_.begin() && std::prev(w)->end > r.begin) return false;
+    const auto rd = std::lower_bound(readable_.begin(), readable_.end(), r.end, starts_before);
+    const auto n = static_cast<std::size_t>(rd - readable_.begin());
+    return n == 0 || reach_[n - 1] <= r.begin;
+}
+
+PlanState::~PlanState() {
+    // Every execution of the plan has completed, so the free needs no ordering.
+    if (storage != nullptr) static_cast<void>(hip_status(hipFreeAsync(storage, hipStreamPerThread)));
+}
+
+// Device memory of `device`, managed memory, mapped pinned host memory, or
+// pageable memory where the device can access it (reference plan.cpp:70-85).
+Status accessible(const void *data, int device, bool pageable) noexcept {
+    hipPointerAttribute_t attr{};
+    const hipError_t e = hipPointerGetAttributes(&attr, data);
+    if (e == hipErrorInvalidValue) {
+        static_cast<void>(hipGetLastError()); // unknown to the runtime: ordinary pageable memory
+        return pageable ? Status::success : Status::invalid_argument;
+    }
+    if (e != hipSuccess) return hip_failure();
+    if (attr.isManaged) return Status::success;
+    switch (attr.type) {
+    case hipMemoryTypeDevice:
+        return attr.device == device ? Status::success : Status::invalid_argument;
+    case hipMemoryTypeManaged:
+    case hipMemoryTypeUnified:
+        return Status::success;
+    case hipMemoryTypeHost:
+        return attr.devicePointer == data ? Status::success : Status::invalid_argument;
+    default:
+        return pageable ? Status::success : Status::invalid_argument;
+    }
+}
+
+Status stream_device(hipStream_t stream, int device) noexcept {
+    int current = 0;
+    hipDevice_t owner = 0;
+    if (hipGetDevice(&current) != hipSuccess || hipStreamGetDevice(stream, &owner) != hipSuccess)
+        return hip_failure();
+    return current == device && owner == device ? Status::success : Status::invalid_argument;
+}
+
+namespace {
+
+Range range_of(const void *data, std::uint64_t size) noexcept {
+    const auto b = reinterpret_cast<std::uintptr_t>(data);
+    return {b, b + static_cast<std::uintptr_t>(size)};
+}
+
+std::uint64_t columns(const ImageSpec &image, TileShape tile) noexcept { return host::ceil_div(image.width, tile.width); }
+std::uint64_t tile_rows(const ImageSpec &image, TileShape tile) noexcept { return host::ceil_div(image.height, tile.height); }
+
+// A map matches the output region exactly (reference plan.cpp:174-193).
+Status check_parameter(const Parameter &p, const ImageSpec &region, std::vector<Range> &readable,
+                       std::vector<const void *> &pointers) {
+    if (p.kind == ParameterKind::scalar) return Status::success;
+    if (p.kind != ParameterKind::map) return Status::invalid_argument;
+    if (const Status st = host::check_view(p.map); st != Status::success) return st;
+    if (p.map.image.type != PixelType::float32 || p.map.image.width != region.width ||
+        p.map.image.height != region.height)
+        return Status::invalid_argument;
+    readable.push_back(range_of(p.map.data, p.map.capacity_bytes));
+    pointers.push_back(p.map.data);
+    return Status::success;
+}
+
+Status check_calibration(const Calibration &c, const ImageSpec &region, std::vector<Range> &readable,
+                         std::vector<const void *> &pointers) {
+    const Status st = check_parameter(c.offset, region, readable, pointers);
+    return st != Status::success ? st : check_parameter(c.gain, region, readable, pointers);
+}
+
+// Repeats the parser's structural checks that keep device reads inside the tile
+// payload (reference plan.cpp:238-279).
+Status check_tile(const ImageMetadata &m, std::uint64_t index) noexcept {
+    const TileMetadata &t = m.tiles[static_cast<std::size_t>(index)];
+    const std::uint64_t across = columns(m.image, m.tile);
+    const std::uint64_t x = index % across * m.tile.width, y = index / across * m.tile.height;
+    const std::uint64_t w = std::min<std::uint64_t>(m.tile.width, m.image.width - x);
+    const std::uint64_t h = std::min<std::uint64_t>(m.tile.height, m.image.height - y);
+    const std::uint64_t s = m.image.type == PixelType::uint16 ? 2 : 4;
+    const std::uint32_t widest = m.image.type == PixelType::uint16 ? 16 : 32;
+    const std::uint64_t raw4 = (w * h * s + 3) / 4 * 4;
+    if (t.width != w || t.height != h || t.offset % 4 != 0 || t.offset > m.file_bytes ||
+        t.length > m.file_bytes - t.offset)
+        return Status::malformed_data;
+    switch (t.mode) {
+    case TileMode::raw:
+        return t.length == raw4 ? Status::success : Status::malformed_data;
+    case TileMode::constant:
+        return t.length == 4 ? Status::success : Status::malformed_data;
+    case TileMode::predict_bitpack:
+        break;
+    default:
+        return Status::malformed_data;
+    }
+    if (t.length > raw4 || t.first_segment > m.segments.size() || h > m.segments.size() - t.first_segment)
+        return Status::malformed_data;
+    for (std::uint64_t r = 0; r < h; ++r) {
+        const SegmentMetadata &g = m.segments[static_cast<std::size_t>(t.first_segment + r)];
+        if (g.bit_width > widest || (m.image.type == PixelType::uint16 && g.seed > 0xffffu) ||
+            g.payload_offset < format::segment_header_bytes || g.payload_offset % 4 != 0 ||
+            g.payload_offset > t.length || g.payload_bytes > t.length - g.payload_offset ||
+            g.payload_bytes != ((w - 1) * g.bit_width + 31) / 32 * 4)
+            return Status::malformed_data;
+    }
+    return Status::success;
+}
+
+// Selects the tiles intersecting the region, rebased into the supplied bytes
+// when the payloads were staged (reference plan.cpp:282-349).
+Status stage_tiles(const CompressedInput &source, const Roi &roi, std::uint32_t image, DecodeDraft &d) {
+    const ImageMetadata &m = *source.metadata;
+    const std::uint64_t across = columns(m.image, m.tile);
+    if (m.tiles.size() != across * tile_rows(m.image, m.tile)) return Status::malformed_data;
+    const std::uint64_t left = roi.x / m.tile.width;
+    const std::uint64_t right = (std::uint64_t{roi.x} + roi.width - 1) / m.tile.width;
+    const std::uint64_t top = roi.y / m.tile.height;
+    const std::uint64_t bottom = (std::uint64_t{roi.y} + roi.height - 1) / m.tile.height;
+    const std::uint64_t selected = (right - left + 1) * (bottom - top + 1);
+    const bool mapped = source.mappings != nullptr || source.mapping_count != 0;
+    if (mapped ? source.mappings == nullptr || source.mapping_count != selected : m.file_bytes > source.bytes.size)
+        return mapped ? Status::invalid_argument : Status::buffer_too_small;
+    std::vector<Range> staged;
+    std::uint64_t k = 0;
+    for (std::uint64_t row = top; row <= bottom; ++row) {
+        for (std::uint64_t col = left; col <= right; ++col, ++k) {
+            const std::uint64_t index = row * across + col;
+            if (const Status st = check_tile(m, index); st != Status::success) return st;
+            const TileMetadata &t = m.tiles[static_cast<std::size_t>(index)];
+            std::uint64_t offset = t.offset;
+            if (mapped) {
+                const TileMapping &map = source.mappings[k];
+                if (map.tile_index != index || map.input_offset % 4 != 0 || map.input_offset > source.bytes.size ||
+                    t.length > source.bytes.size - map.input_offset)
+                    return Status::invalid_argument;
+                offset = map.input_offset;
+                staged.push_back({static_cast<std::uintptr_t>(offset), static_cast<std::uintptr_t>(offset + t.length)});
+            }
+            tsk_decode_task task{};
+            task.offset = offset;
+            task.tile_index = index;
+            task.image = image;
+            task.rows = TSK_NO_ROWS;
+            task.words = static_cast<std::uint32_t>(t.length / 4);
+            task.mode = static_cast<std::uint32_t>(t.mode);
+            if (t.mode == TileMode::predict_bitpack) {
+                task.rows = static_cast<std::uint32_t>(d.rows.size());
+                for (std::uint64_t r = 0; r < t.height; ++r) {
+                    const SegmentMetadata &g = m.segments[static_cast<std::size_t>(t.first_segment + r)];
+                    // Word offset of the row descriptor within the tile payload.
+                    d.rows.push_back(static_cast<std::uint16_t>((g.payload_offset - format::segment_header_bytes) / 4));
+                }
+            }
+            d.tasks.push_back(task);
+        }
+    }
+    std::sort(staged.begin(), staged.end(), [](const Range &a, const Range &b) { return a.begin < b.begin; });
+    for (std::size_t i = 1; i < staged.size(); ++i)
+        if (staged[i].begin < staged[i - 1].end) return Status::invalid_argument;
+    d.tiles_per_image.push_back(selected);
+    return Status::success;
+}
+
+} // namespace
+
+// One decode item (reference plan.cpp:351-465).
+Status draft_decode_item(const BatchItem &item, Operation operation, bool require_one_type, DecodeDraft &d) {
+    const CompressedInput &source = item.compressed_input;
+    const bool trusted = source.encoder_result != nullptr;
+    if (trusted ? source.metadata != nullptr || source.mappings != nullptr || source.mapping_count != 0
+                : source.metadata == nullptr)
+        return Status::invalid_argument;
+    const ImageSpec image = trusted ? source.trusted_image : source.metadata->image;
+    const TileShape shape = trusted ? source.trusted_tile : source.metadata->tile;
+    std::uint64_t bound = 0;
+    if (const Status st = compress_bound(image, shape, bound); st != Status::success) return st;
+    if (d.first) {
+        d.type = image.type;
+        d.shape = shape;
+        d.first = false;
+    }
+    if (require_one_type && image.type != d.type) return Status::invalid_argument;
+    d.mixed_shapes = d.mixed_shapes || shape.width != d.shape.width || shape.height != d.shape.height;
+
+    const ImageView &output = item.output;
+    if (const Status st = host::check_view(host::as_const(output)); st != Status::success) return st;
+    Roi roi{0, 0, image.width, image.height};
+    if (operation != Operation::decompress) {
+        roi = item.roi;
+        if (roi.width == 0 || roi.height == 0 || std::uint64_t{roi.x} + roi.width > image.width ||
+            std::uint64_t{roi.y} + roi.height > image.height)
+            return Status::invalid_roi;
+    }
+    const bool calibrated = operation == Operation::decode_calibrate;
+    if (output.image.type != (calibrated ? PixelType::float32 : image.type) || output.image.width != roi.width ||
+        output.image.height != roi.height)
+        return Status::invalid_argument;
+    if (calibrated)
+        if (const Status st = check_calibration(item.calibration, output.image, d.readable, d.pointers);
+            st != Status::success)
+            return st;
+    if (const Status st = byte_range(source.bytes.data, source.bytes.size); st != Status::success) return st;
+    if (source.bytes.data == nullptr || reinterpret_cast<std::uintptr_t>(source.bytes.data) % 4 != 0)
+        return Status::invalid_argument;
+
+    const auto index = static_cast<std::uint32_t>(d.images.size());
+    if (trusted) {
+        // Modes and offsets are known only on the device, after the encoder ran.
+        if (reinterpret_cast<std::uintptr_t>(source.encoder_result) % alignof(DeviceResult) != 0)
+            return Status::invalid_argument;
+        if (source.bytes.size < bound) return Status::buffer_too_small;
+        const std::uint64_t across = columns(image, shape);
+        std::uint64_t n = 0;
+        for (std::uint64_t row = roi.y / shape.height; row <= (std::uint64_t{roi.y} + roi.height - 1) / shape.height; ++row)
+            for (std::uint64_t col = roi.x / shape.width; col <= (std::uint64_t{roi.x} + roi.width - 1) / shape.width;
+                 ++col, ++n) {
+                tsk_decode_task task{};
+                task.tile_index = row * across + col;
+                task.image = index;
+                task.rows = TSK_NO_ROWS;
+                d.tasks.push_back(task);
+            }
+        d.tiles_per_image.push_back(n);
+        d.readable.push_back(range_of(source.encoder_result, sizeof(DeviceResult)));
+        d.pointers.push_back(source.encoder_result);
+    } else {
+        if (const Status st = stage_tiles(source, roi, index, d); st != Status::success) return st;
+    }
+
+    tsk_decode_image rec{};
+    rec.src = static_cast<const std::uint8_t *>(source.bytes.data);
+    rec.out = static_cast<std::uint8_t *>(output.data);
+    rec.producer = source.encoder_result;
+    rec.src_size = source.bytes.size;
+    rec.out_pitch = output.pitch_bytes;
+    rec.width = image.width;
+    rec.height = image.height;
+    rec.roi_x = roi.x;
+    rec.roi_y = roi.y;
+    rec.roi_w = roi.width;
+    rec.roi_h = roi.height;
+    rec.tile_columns = static_cast<std::uint32_t>(columns(image, shape));
+    rec.tile_w = shape.width;
+    rec.tile_h = shape.height;
+    rec.type = static_cast<std::uint8_t>(image.type);
+    rec.calibrated = calibrated ? 1 : 0;
+    rec.offset = 0.0F;
+    rec.gain = 1.0F;
+    if (calibrated) {
+        const Parameter &o = item.calibration.offset, &g = item.calibration.gain;
+        rec.offset = o.kind == ParameterKind::scalar ? o.scalar : 0.0F;
+        rec.gain = g.kind == ParameterKind::scalar ? g.scalar : 1.0F;
+        if (o.kind == ParameterKind::map) {
+            rec.offset_map = static_cast<const std::uint8_t *>(o.map.data);
+            rec.offset_pitch = o.map.pitch_bytes;
+        }
+        if (g.kind == ParameterKind::map) {
+            rec.gain_map = static_cast<const std::uint8_t *>(g.map.data);
+            rec.gain_pitch = g.map.pitch_bytes;
+        }
+    }
+    d.images.push_back(rec);
+    d.readable.push_back(range_of(source.bytes.data, source.bytes.size));
+    d.writable.push_back(range_of(output.data, output.capacity_bytes));
+    d.pointers.push_back(source.bytes.data);
+    d.pointers.push_back(output.data);
+    return Status::success;
+}
+
+namespace {
+
+struct Draft {
+    PixelType type = PixelType::uint16;
+    TileShape shape;
+    DecodeDraft decode;
+    std::vector<tsk_encode_image> encode;
+    std::vector<tsk_calibrate_image> calibrate;
+    std::vector<std::uint64_t> first_rows;
+    std::uint64_t total_rows = 0;
+    std::vector<std::uint64_t> tiles; // tiles per image, for windows
+    std::vector<Range> writable;
+    std::vector<Range> readable;
+    std::vector<const void *> pointers;
+};
+
+Status draft_compress(const BatchRequest &request, Draft &d) {
+    d.type = request.items[0].input.image.type;
+    d.shape = request.tile;
+    for (std::uint64_t i = 0; i < request.count; ++i) {
+        const BatchItem &item = request.items[i];
+        if (const Status st = host::check_view(item.input); st != Status::success) return st;
+        if (item.input.image.type != d.type) return Status::invalid_argument;
+        std::uint64_t bound = 0;
+        if (const Status st = compress_bound(item.input.image, request.tile, bound); st != Status::success) return st;
+        const MutableBytes &out = item.compressed_output;
+        if (const Status st = byte_range(out.data, out.size); st != Status::success) return st;
+        if (out.data == nullptr || reinterpret_cast<std::uintptr_t>(out.data) % 4 != 0) return Status::invalid_argument;
+        if (out.size < bound) return Status::buffer_too_small;
+        tsk_encode_image rec{};
+        rec.in = static_cast<const std::uint8_t *>(item.input.data);
+        rec.out = static_cast<std::uint8_t *>(out.data);
+        rec.in_pitch = item.input.pitch_bytes;
+        rec.tile_columns = static_cast<std::uint32_t>(columns(item.input.image, request.tile));
+        rec.tile_count = columns(item.input.image, request.tile) * tile_rows(item.input.image, request.tile);
+        rec.width = item.input.image.width;
+        rec.height = item.input.image.height;
+        rec.tile_w = request.tile.width;
+        rec.tile_h = request.tile.height;
+        rec.type = static_cast<std::uint32_t>(item.input.image.type);
+        d.encode.push_back(rec);
+        d.tiles.push_back(rec.tile_count);
+        d.readable.push_back(range_of(item.input.data, item.input.capacity_bytes));
+        d.writable.push_back(range_of(out.data, out.size));
+        d.pointers.push_back(item.input.data);
+        d.pointers.push_back(out.data);
+    }
+    return Status::success;
+}
+
+Status draft_calibrate(const BatchRequest &request, Draft &d) {
+    d.type = request.items[0].input.image.type;
+    for (std::uint64_t i = 0; i < request.count; ++i) {
+        const BatchItem &item = request.items[i];
+        Status st = host::check_view(item.input);
+        if (st == Status::success) st = host::check_view(host::as_const(item.output));
+        if (st != Status::success) return st;
+        if (item.input.image.type != d.type || item.output.image.type != PixelType::float32 ||
+            item.output.image.width != item.input.image.width || item.output.image.height != item.input.image.height)
+            return Status::invalid_argument;
+        if ((st = check_calibration(item.calibration, item.output.image, d.readable, d.pointers)) != Status::success)
+            return st;
+        tsk_calibrate_image rec{};
+        rec.in = static_cast<const std::uint8_t *>(item.input.data);
+        rec.out = static_cast<std::uint8_t *>(item.output.data);
+        rec.in_pitch = item.input.pitch_bytes;
+        rec.out_pitch = item.output.pitch_bytes;
+        rec.width = item.input.image.width;
+        rec.height = item.input.image.height;
+        const Parameter &o = item.calibration.offset, &g = item.calibration.gain;
+        rec.offset = o.kind == ParameterKind::scalar ? o.scalar : 0.0F;
+        rec.gain = g.kind == ParameterKind::scalar ? g.scalar : 1.0F;
+        if (o.kind == ParameterKind::map) {
+            rec.offset_map = static_cast<const std::uint8_t *>(o.map.data);
+            rec.offset_pitch = o.map.pitch_bytes;
+        }
+        if (g.kind == ParameterKind::map) {
+            rec.gain_map = static_cast<const std::uint8_t *>(g.map.data);
+            rec.gain_pitch = g.map.pitch_bytes;
+        }
+        d.calibrate.push_back(rec);
+        d.first_rows.push_back(d.total_rows);
+        d.total_rows += item.input.image.height;
+        d.readable.push_back(range_of(item.input.data, item.input.capacity_bytes));
+        d.writable.push_back(range_of(item.output.data, item.output.capacity_bytes));
+        d.pointers.push_back(item.input.data);
+        d.pointers.push_back(item.output.data);
+    }
+    return Status::success;
+}
+
+Status draft_decode(const BatchRequest &request, Draft &d) {
+    for (std::uint64_t i = 0; i < request.count; ++i)
+        if (const Status st = draft_decode_item(request.items[i], request.operation, true, d.decode); st != Status::success)
+            return st;
+    d.type = d.decode.type;
+    d.shape = d.decode.mixed_shapes ? TileShape{64, 64} : d.decode.shape;
+    d.tiles = d.decode.tiles_per_image;
+    auto move_into = [](auto &from, auto &to) { to.insert(to.end(), from.begin(), from.end()); };
+    move_into(d.decode.writable, d.writable);
+    move_into(d.decode.readable, d.readable);
+    move_into(d.decode.pointers, d.pointers);
+    return Status::success;
+}
+
+// Splits the concatenated tiles of all images into windows of at most
+// window_tiles tiles (reference plan.cpp:468-491).
+struct WindowPlan {
+    std::vector<tsk_window_entry> entries;
+    std::vector<std::pair<std::size_t, EncodeWindow>> windows; // first entry, window
+};
+
+WindowPlan split_windows(const std::vector<std::uint64_t> &tiles) {
+    WindowPlan plan;
+    std::uint32_t used = 0;
+    for (std::size_t image = 0; image < tiles.size(); ++image) {
+        for (std::uint64_t start = 0; start < tiles[image];) {
+            if (plan.windows.empty() || used == window_tiles) {
+                plan.windows.push_back({plan.entries.size(), {}});
+                used = 0;
+            }
+            const auto take = static_cast<std::uint32_t>(std::min<std::uint64_t>(tiles[image] - start, window_tiles - used));
+            std::uint32_t flags = start == 0 ? 0u : 1u;
+            if (start + take == tiles[image]) flags |= 2u;
+            plan.entries.push_back({static_cast<std::uint32_t>(image), used, take, flags, start});
+            EncodeWindow &w = plan.windows.back().second;
+            ++w.entry_count;
+            w.tile_count += take;
+            used += take;
+            start += take;
+        }
+    }
+    return plan;
+}
+
+template <typename T> bool upload(void *storage, std::uint64_t at, const std::vector<T> &v, hipStream_t stream) {
+    return v.empty() || hipMemcpyAsync(static_cast<std::uint8_t *>(storage) + at, v.data(), v.size() * sizeof(T),
+                                       hipMemcpyHostToDevice, stream) == hipSuccess;
+}
+
+Status build(const BatchRequest &request, PlanState &plan, hipStream_t stream) {
+    plan.operation = request.operation;
+    plan.count = request.count;
+    if (request.count == 0) return Status::success;
+
+    Draft d;
+    Status st = request.operation == Operation::compress    ? draft_compress(request, d)
+                : request.operation == Operation::calibrate ? draft_calibrate(request, d)
+                                                            : draft_decode(request, d);
+    if (st != Status::success) return st;
+    plan.type = d.type;
+    if (!plan.ranges.assign(std::move(d.writable), std::move(d.readable))) return Status::invalid_argument;
+
+    int pageable = 0;
+    if (hipGetDevice(&plan.device) != hipSuccess ||
+        hipDeviceGetAttribute(&pageable, hipDeviceAttributePageableMemoryAccess, plan.device) != hipSuccess)
+        return hip_failure();
+    plan.pageable = pageable != 0;
+    if ((st = stream_device(stream, plan.device)) != Status::success) return st;
+    for (const void *p : d.pointers)
+        if ((st = accessible(p, plan.device, plan.pageable)) != Status::success) return st;
+
+    WindowPlan windows = split_windows(d.tiles);
+    plan.spec = {0, 0, d.shape, 0};
+    for (const auto &[first, w] : windows.windows) {
+        plan.spec.image_count = std::max<std::uint64_t>(plan.spec.image_count, w.entry_count);
+        plan.spec.tile_count = std::max<std::uint64_t>(plan.spec.tile_count, w.tile_count);
+    }
+    if ((st = host::workspace_layout(plan.spec, plan.layout)) != Status::success) return st;
+
+    std::uint64_t total = 0;
+    const auto place = [&total](std::uint64_t count, std::uint64_t size, std::uint64_t &at) {
+        std::uint64_t bytes = 0;
+        return host::round_up(total, 16, at) && host::mul(count, size, bytes) && host::add(at, bytes, total);
+    };
+    std::uint64_t dec_images_at = 0, tasks_at = 0, rows_at = 0, enc_images_at = 0, entries_at = 0, cal_at = 0, first_at = 0;
+    if (!place(d.decode.images.size(), sizeof(tsk_decode_image), dec_images_at) ||
+        !place(d.decode.tasks.size(), sizeof(tsk_decode_task), tasks_at) ||
+        !place(d.decode.rows.size(), sizeof(std::uint16_t), rows_at) ||
+        !place(d.encode.size(), sizeof(tsk_encode_image), enc_images_at) ||
+        !place(windows.entries.size(), sizeof(tsk_window_entry), entries_at) ||
+        !place(d.calibrate.size(), sizeof(tsk_calibrate_image), cal_at) ||
+        !place(d.first_rows.size(), sizeof(std::uint64_t), first_at))
+        return Status::size_overflow;
+    if (hipMallocAsync(&plan.storage, static_cast<std::size_t>(std::max<std::uint64_t>(total, 16)), stream) != hipSuccess) {
+        plan.storage = nullptr;
+        return hip_failure();
+    }
+    auto *base = static_cast<std::uint8_t *>(plan.storage);
+    plan.decode_images = reinterpret_cast<const tsk_decode_image *>(base + dec_images_at);
+    plan.tasks = reinterpret_cast<const tsk_decode_task *>(base + tasks_at);
+    plan.rows = reinterpret_cast<const std::uint16_t *>(base + rows_at);
+    plan.task_count = d.decode.tasks.size();
+    plan.encode_images = reinterpret_cast<const tsk_encode_image *>(base + enc_images_at);
+    plan.calibrate_images = reinterpret_cast<const tsk_calibrate_image *>(base + cal_at);
+    plan.first_rows = reinterpret_cast<const std::uint64_t *>(base + first_at);
+    plan.total_rows = d.total_rows;
+    for (const auto &[first, w] : windows.windows) {
+        EncodeWindow placed = w;
+        placed.entries = reinterpret_cast<const tsk_window_entry *>(base + entries_at) + first;
+        plan.windows.push_back(placed);
+    }
+    const std::uint32_t op = request.operation == Operation::compress    ? 0u
+                             : request.operation == Operation::calibrate ? 2u
+                             : request.operation == Operation::decode_calibrate ? 3u
+                                                                                 : 1u;
+    if (tsk_preload(op) != hipSuccess) return hip_failure();
+    if (!upload(plan.storage, dec_images_at, d.decode.images, stream) ||
+        !upload(plan.storage, tasks_at, d.decode.tasks, stream) || !upload(plan.storage, rows_at, d.decode.rows, stream) ||
+        !upload(plan.storage, enc_images_at, d.encode, stream) ||
+        !upload(plan.storage, entries_at, windows.entries, stream) ||
+        !upload(plan.storage, cal_at, d.calibrate, stream) || !upload(plan.storage, first_at, d.first_rows, stream) ||
+        hipStreamSynchronize(stream) != hipSuccess)
+        return hip_failure();
+    plan.host_images = std::move(d.decode.images);
+    plan.host_tasks = std::move(d.decode.tasks);
+    plan.host_rows = std::move(d.decode.rows);
+    return Status::success;
+}
+
+} // namespace
+
+// Host checks of every execution call, before anything is enqueued
+// (reference plan.cpp:627-674).
+Status check_execution(const PlanState *plan, Operation operation, MutableBytes workspace, ResultBuffer results,
+                       hipStream_t stream) noexcept {
+    if (plan == nullptr || plan->operation != operation) return Status::invalid_argument;
+    if (plan->count == 0) return Status::success;
+    std::uint64_t result_bytes = 0;
+    if (!host::mul(results.count, sizeof(DeviceResult), result_bytes)) return Status::size_overflow;
+    Status st = byte_range(workspace.data, workspace.size);
+    if (st == Status::success) st = byte_range(results.data, result_bytes);
+    if (st != Status::success) return st;
+    if (workspace.size < plan->layout.total || results.count < plan->count) return Status::buffer_too_small;
+    if (reinterpret_cast<std::uintptr_t>(workspace.data) % workspace_alignment != 0 ||
+        reinterpret_cast<std::uintptr_t>(results.data) % alignof(DeviceResult) != 0)
+        return Status::invalid_argument;
+    if (!plan->ranges.disjoint(range_of(workspace.data, workspace.size)) ||
+        !plan->ranges.disjoint(range_of(results.data, result_bytes)) ||
+        host::overlaps(workspace.data, workspace.size, results.data, result_bytes))
+        return Status::invalid_argument;
+    st = stream_device(stream, plan->device);
+    // An empty workspace is never touched, so its pointer need not be device memory.
+    if (st == Status::success && workspace.size != 0) st = accessible(workspace.data, plan->device, plan->pageable);
+    if (st == Status::success) st = accessible(results.data, plan->device, plan->pageable);
+    // A pending error from the caller's earlier work would be reported as ours:
+    // leave it for the caller and enqueue nothing.
+    if (st == Status::success && hipPeekAtLastError() != hipSuccess) st = Status::hip_error;
+    return st;
+}
+
+Status run_decode(const PlanState &plan, DeviceResult *results, hipStream_t stream, const DecodeShape &shape) noexcept {
+    const auto count = static_cast<std::uint32_t>(plan.count);
+    if (tsk_results_begin(results, count, stream) != hipSuccess) return hip_failure();
+    tsk_decode_args a{};
+    a.images = plan.decode_images;
+    a.tasks = plan.tasks;
+    a.rows = plan.rows;
+    a.results = results;
+    a.task_count = plan.task_count;
+    a.image_count = count;
+    const bool calibrated = plan.operation == Operation::decode_calibrate;
+    a.kind = plan.type == PixelType::uint16 ? (calibrated ? TSK_KIND_U16_CAL : TSK_KIND_U16)
+                                            : (calibrated ? TSK_KIND_F32_CAL : TSK_KIND_F32);
+    a.group = shape.group;
+    a.block = shape.block;
+    const std::uint64_t grid = shape.grid != 0 ? shape.grid : std::min<std::uint64_t>(plan.task_count, 1u << 20);
+    a.grid = static_cast<std::uint32_t>(std::max<std::uint64_t>(grid, 1));
+    if (tsk_decode(&a, stream) != hipSuccess) return hip_failure();
+    if (tsk_results_complete(plan.decode_images, count, results, stream) != hipSuccess) return hip_failure();
+    return Status::success;
+}
+
+} // namespace detail
+
+BatchPlan::BatchPlan(std::unique_ptr<Impl> impl) noexcept : impl_(std::move(impl)) {}
+BatchPlan::~BatchPlan() = default;
+BatchPlan::BatchPlan(BatchPlan &&) noexcept = default;
+BatchPlan &BatchPlan::operator=(BatchPlan &&) noexcept = default;
+
+WorkspaceSpec BatchPlan::workspace_spec() const noexcept { return impl_ != nullptr ? impl_->spec : WorkspaceSpec{}; }
+
+Status prepare_batch(const BatchRequest &request, std::unique_ptr<BatchPlan> &plan, hipStream_t stream) noexcept {
+    if (request.count > UINT32_MAX) return Status::resource_limit;
+    if (request.count != 0 && request.items == nullptr) return Status::invalid_argument;
+    if (request.operation > Operation::decode_calibrate) return Status::invalid_argument;
+    try {
+        auto impl = std::make_unique<BatchPlan::Impl>();
+        if (const Status st = detail::build(request, *impl, stream); st != Status::success) return st;
+        plan.reset(new BatchPlan(std::move(impl)));
+        return Status::success;
+    } catch (const std::bad_alloc &) {
+        return Status::resource_limit;
+    } catch (const std::length_error &) {
+        return Status::resource_limit;
+    }
+}
+
+namespace {
+
+Status decode_entry(const BatchPlan &plan, Operation op, MutableBytes workspace, ResultBuffer results,
+                    hipStream_t stream) noexcept {
+    const detail::PlanState *state = plan.impl();
+    const Status st = detail::check_execution(state, op, workspace, results, stream);
+    if (st != Status::success || state->count == 0) return st;
+    return detail::run_decode(*state, results.data, stream, state->shape);
+}
+
+} // namespace
+
+Status decompress_async(const BatchPlan &plan, MutableBytes workspace, ResultBuffer results, hipStream_t stream) noexcept {
+    return decode_entry(plan, Operation::decompress, workspace, results, stream);
+}
+
+Status decode_roi_async(const BatchPlan &plan, MutableBytes workspace, ResultBuffer results, hipStream_t stream) noexcept {
+    return decode_entry(plan, Operation::decode_roi, workspace, results, stream);
+}
+
+Status decode_calibrate_async(const BatchPlan &plan, MutableBytes workspace, ResultBuffer results,
+                              hipStream_t stream) noexcept {
+    return decode_entry(plan, Operation::decode_calibrate, workspace, results, stream);
+}
+
+Status compress_async(const BatchPlan &plan, MutableBytes workspace, ResultBuffer results, hipStream_t stream) noexcept {
+    const detail::PlanState *state = plan.impl();
+    const Status st = detail::check_execution(state, Operation::compress, workspace, results, stream);
+    if (st != Status::success || state->count == 0) return st;
+    auto *ws = static_cast<std::uint8_t *>(workspace.data);
+    tsk_encode_args a{};
+    a.images = state->encode_images;
+    a.results = results.data;
+    a.rows = ws + state->layout.rows;
+    a.tiles = reinterpret_cast<std::uint32_t *>(ws + state->layout.tiles);
+    a.offsets = reinterpret_cast<std::uint64_t *>(ws + state->layout.offsets);
+    a.carry = reinterpret_cast<std::uint64_t *>(ws + state->layout.images);
+    a.max_rows = state->spec.tile.height;
+    a.type = static_cast<std::uint32_t>(state->type);
+    for (std::size_t k = 0; k < state->windows.size(); ++k) {
+        const detail::EncodeWindow &w = state->windows[k];
+        a.entries = w.entries;
+        a.entry_count = w.entry_count;
+        a.tile_count = w.tile_count;
+        a.parity = static_cast<std::uint32_t>(k & 1u);
+        if (tsk_encode_analyze(&a, stream) != hipSuccess || tsk_encode_scan(&a, stream) != hipSuccess ||
+            tsk_encode_pack(&a, stream) != hipSuccess || tsk_encode_finish(&a, stream) != hipSuccess)
+            return detail::hip_failure();
+    }
+    return Status::success;
+}
+
+Status calibrate_async(const BatchPlan &plan, MutableBytes workspace, ResultBuffer results, hipStream_t stream) noexcept {
+    const detail::PlanState *state = plan.impl();
+    const Status st = detail::check_execution(state, Operation::calibrate, workspace, results, stream);
+    if (st != Status::success || state->count == 0) return st;
+    tsk_calibrate_args a{};
+    a.images = state->calibrate_images;
+    a.first_rows = state->first_rows;
+    a.results = results.data;
+    a.total_rows = state->total_rows;
+    a.image_count = static_cast<std::uint32_t>(state->count);
+    a.type = static_cast<std::uint32_t>(state->type);
+    return detail::hip_status(tsk_calibrate(&a, stream));
+}
+
+// DeviceBuffer: stream-ordered allocations (reference src/cuda/buffer.cpp:8-61).
+DeviceBuffer::~DeviceBuffer() {
+    if (data_ != nullptr) static_cast<void>(detail::hip_status(hipFreeAsync(data_, hipStreamPerThread)));
+}
+
+DeviceBuffer::DeviceBuffer(DeviceBuffer &&other) noexcept
+    : data_(std::exchange(other.data_, nullptr)), size_(std::exchange(other.size_, 0)) {}
+
+DeviceBuffer &DeviceBuffer::operator=(DeviceBuffer &&other) noexcept {
+    if (this != &other) {
+        DeviceBuffer old(std::move(*this));
+        data_ = std::exchange(other.data_, nullptr);
+        size_ = std::exchange(other.size_, 0);
+    }
+    return *this;
+}
+
+Status DeviceBuffer::allocate(std::uint64_t bytes, hipStream_t stream, DeviceBuffer &buffer) noexcept {
+    if (bytes > max_bytes) return Status::size_overflow;
+    DeviceBuffer fresh;
+    if (bytes != 0) {
+        if (hipMallocAsync(&fresh.data_, static_cast<std::size_t>(bytes), stream) != hipSuccess) {
+            fresh.data_ = nullptr;
+            return detail::hip_failure();
+        }
+        fresh.size_ = bytes;
+        if (reinterpret_cast<std::uintptr_t>(fresh.data_) % workspace_alignment != 0) {
+            static_cast<void>(fresh.release(stream));
+            return Status::hip_error;
+        }
+    }
+    const Status st = buffer.release(stream);
+    buffer = std::move(fresh);
+    return st;
+}
+
+Status DeviceBuffer::release(hipStream_t stream) noexcept {
+    if (data_ == nullptr) return Status::success;
+    const hipError_t e = hipFreeAsync(data_, stream);
+    data_ = nullptr;
+    size_ = 0;
+    return detail::hip_status(e);
+}
+
+Status synchronize(hipStream_t stream) noexcept { return detail::hip_status(hipStreamSynchronize(stream)); }
+
+} // namespace tilestream::hip
