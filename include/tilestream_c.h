@@ -119,6 +119,32 @@ uint32_t ts_synchronize(void *stream);
 /* group: columns per lane (1,2,4,8,16); block: 64..512; grid: 0 = one workgroup per tile. */
 uint32_t ts_plan_set_decode_shape(ts_plan *plan, uint32_t group, uint32_t block, uint32_t grid);
 
+/* ---- extension: batched persistent decode (tilestream/batch.h) ---------
+ * No counterpart in the reference.  Items may mix pixel types, tile shapes,
+ * regions and fused calibration; operations[i] is 1 decompress, 2 decode_roi
+ * or 4 decode_calibrate. */
+typedef struct ts_queue ts_queue;
+typedef struct { uint32_t group, block, grid, chunk; } ts_launch_shape;
+typedef struct {
+    uint64_t images, tasks, decoded_bytes, output_bytes, payload_bytes, map_bytes;
+} ts_queue_stats;
+uint32_t ts_queue_prepare(const ts_batch_item *items, const uint32_t *operations, uint64_t count,
+                          ts_queue **queue, void *stream);
+void ts_queue_destroy(ts_queue *queue);
+uint64_t ts_queue_workspace_bytes(const ts_queue *queue);
+void ts_queue_get_stats(const ts_queue *queue, ts_queue_stats *out);
+void ts_queue_get_shape(const ts_queue *queue, ts_launch_shape *out);
+uint32_t ts_queue_set_shape(ts_queue *queue, const ts_launch_shape *shape);
+uint32_t ts_queue_run(ts_queue *queue, ts_mutable_bytes workspace, ts_device_result *results,
+                      uint64_t result_count, void *stream);
+uint32_t ts_queue_capture(ts_queue *queue, ts_mutable_bytes workspace, ts_device_result *results,
+                          uint64_t result_count, void *stream);
+uint32_t ts_queue_launch(ts_queue *queue, void *stream);
+/* Times each candidate (median of `repeats`), keeps the fastest, writes times in ms. */
+uint32_t ts_queue_autotune(ts_queue *queue, ts_mutable_bytes workspace, ts_device_result *results,
+                           uint64_t result_count, void *stream, const ts_launch_shape *candidates,
+                           uint64_t candidate_count, uint32_t repeats, float *milliseconds);
+
 /* ---- HIP runtime helpers for hosts without their own HIP binding --------
  * These return hipError_t values (0 = hipSuccess). */
 int ts_hip_malloc(uint64_t bytes, void **data);

@@ -146,6 +146,8 @@ int tsk_results_begin(void *results, uint32_t count, void *stream);
 int tsk_results_complete(const tsk_decode_image *images, uint32_t count, void *results,
                          void *stream);
 int tsk_decode(const tsk_decode_args *args, void *stream);
+/* Resident workgroups per CU of a decode kernel variant (for persistent grids). */
+int tsk_decode_occupancy(uint32_t kind, uint32_t group, uint32_t block, int *blocks);
 int tsk_encode_analyze(const tsk_encode_args *args, void *stream);
 int tsk_encode_scan(const tsk_encode_args *args, void *stream);
 int tsk_encode_pack(const tsk_encode_args *args, void *stream);

@@ -119,6 +119,15 @@ class BatchRequest(C.Structure):
                 ("tile", TileShape)]
 
 
+class LaunchShape(C.Structure):
+    _fields_ = [("group", C.c_uint32), ("block", C.c_uint32), ("grid", C.c_uint32), ("chunk", C.c_uint32)]
+
+
+class QueueStats(C.Structure):
+    _fields_ = [("images", C.c_uint64), ("tasks", C.c_uint64), ("decoded_bytes", C.c_uint64),
+                ("output_bytes", C.c_uint64), ("payload_bytes", C.c_uint64), ("map_bytes", C.c_uint64)]
+
+
 RECORDS = {
     "ts_image_spec": ImageSpec, "ts_tile_shape": TileShape, "ts_roi": Roi,
     "ts_const_bytes": ConstBytes, "ts_mutable_bytes": MutableBytes,
@@ -128,7 +137,7 @@ RECORDS = {
     "ts_tile_metadata": TileMetadata, "ts_segment_metadata": SegmentMetadata,
     "ts_parameter": Parameter, "ts_calibration": Calibration, "ts_tile_mapping": TileMapping,
     "ts_compressed_input": CompressedInput, "ts_batch_item": BatchItem,
-    "ts_batch_request": BatchRequest,
+    "ts_batch_request": BatchRequest, "ts_launch_shape": LaunchShape, "ts_queue_stats": QueueStats,
 }
 
 _u32, _u64, _vp, _int = C.c_uint32, C.c_uint64, C.c_void_p, C.c_int
@@ -160,6 +169,17 @@ PROTOTYPES = {
     "ts_buffer_release": (_u32, [_vp, _vp]),
     "ts_synchronize": (_u32, [_vp]),
     "ts_plan_set_decode_shape": (_u32, [_vp, _u32, _u32, _u32]),
+    "ts_queue_prepare": (_u32, [_P(BatchItem), _P(_u32), _u64, _P(_vp), _vp]),
+    "ts_queue_destroy": (None, [_vp]),
+    "ts_queue_workspace_bytes": (_u64, [_vp]),
+    "ts_queue_get_stats": (None, [_vp, _P(QueueStats)]),
+    "ts_queue_get_shape": (None, [_vp, _P(LaunchShape)]),
+    "ts_queue_set_shape": (_u32, [_vp, _P(LaunchShape)]),
+    "ts_queue_run": (_u32, [_vp, MutableBytes, _vp, _u64, _vp]),
+    "ts_queue_capture": (_u32, [_vp, MutableBytes, _vp, _u64, _vp]),
+    "ts_queue_launch": (_u32, [_vp, _vp]),
+    "ts_queue_autotune": (_u32, [_vp, MutableBytes, _vp, _u64, _vp, _P(LaunchShape), _u64, _u32,
+                                 _P(C.c_float)]),
     "ts_hip_malloc": (_int, [_u64, _P(_vp)]),
     "ts_hip_free": (_int, [_vp]),
     "ts_hip_host_malloc": (_int, [_u64, _P(_vp)]),

@@ -16,9 +16,9 @@ import numpy as np
 
 from . import _native as N
 from ._native import (BatchItem, BatchRequest, Calibration, CompressedInput, ConstBytes,
-                      ConstImageView, DeviceResult, ImageSpec, ImageView, MutableBytes, Parameter,
-                      ParseLimits, Roi, SegmentMetadata, TileMapping, TileMetadata, TileShape,
-                      WorkspaceSpec)
+                      ConstImageView, DeviceResult, ImageSpec, ImageView, LaunchShape, MutableBytes,
+                      Parameter, ParseLimits, QueueStats, Roi, SegmentMetadata, TileMapping,
+                      TileMetadata, TileShape, WorkspaceSpec)
 
 MAX_BYTES = (1 << 63) - 1
 
@@ -400,3 +400,73 @@ def scalar(value: float) -> Parameter:
 
 def map_parameter(view: ConstImageView) -> Parameter:
     return Parameter(int(ParameterKind.map), 0.0, view)
+
+
+# ---- batched persistent decode (extension) ------------------------------------------
+
+class DecodeQueue:
+    """A prepared mixed batch drained by one persistent decode launch."""
+
+    def __init__(self, handle: int, keepalive):
+        self.handle = handle
+        self._keep = keepalive
+        self.count = len(keepalive[1])
+
+    @classmethod
+    def prepare(cls, items: Sequence[BatchItem], operations: Sequence[int], stream=None):
+        arr = (BatchItem * max(1, len(items)))(*items)
+        ops = (C.c_uint32 * max(1, len(items)))(*[int(o) for o in operations])
+        handle = C.c_void_p(0)
+        st = Status(N.lib().ts_queue_prepare(arr, ops, len(items), C.byref(handle), _stream(stream)))
+        if st != Status.success:
+            return st, None
+        return st, cls(handle.value, (arr, items, ops))
+
+    def workspace_bytes(self) -> int:
+        return int(N.lib().ts_queue_workspace_bytes(self.handle))
+
+    def stats(self) -> QueueStats:
+        s = QueueStats()
+        N.lib().ts_queue_get_stats(self.handle, C.byref(s))
+        return s
+
+    @property
+    def shape(self) -> LaunchShape:
+        s = LaunchShape()
+        N.lib().ts_queue_get_shape(self.handle, C.byref(s))
+        return s
+
+    def set_shape(self, group=8, block=256, grid=0, chunk=1) -> Status:
+        return Status(N.lib().ts_queue_set_shape(self.handle, C.byref(LaunchShape(group, block, grid, chunk))))
+
+    def _ws(self, workspace):
+        return MutableBytes(workspace.ptr, workspace.size) if workspace is not None else MutableBytes(None, 0)
+
+    def run(self, workspace, results, stream=None) -> Status:
+        return Status(N.lib().ts_queue_run(self.handle, self._ws(workspace), results.ptr, self.count,
+                                           _stream(stream)))
+
+    def capture(self, workspace, results, stream) -> Status:
+        return Status(N.lib().ts_queue_capture(self.handle, self._ws(workspace), results.ptr, self.count,
+                                               _stream(stream)))
+
+    def launch(self, stream) -> Status:
+        return Status(N.lib().ts_queue_launch(self.handle, _stream(stream)))
+
+    def autotune(self, workspace, results, stream, candidates, repeats=5):
+        arr = (LaunchShape * len(candidates))(*[LaunchShape(*c) for c in candidates])
+        ms = (C.c_float * len(candidates))()
+        st = Status(N.lib().ts_queue_autotune(self.handle, self._ws(workspace), results.ptr, self.count,
+                                              _stream(stream), arr, len(candidates), repeats, ms))
+        return st, list(ms)
+
+    def close(self) -> None:
+        if self.handle:
+            N.lib().ts_queue_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -30,7 +30,7 @@ HIP_FLAGS = ["-x", "hip", "--offload-arch=" + ARCH, "-fno-gpu-rdc",
 
 HIP_SOURCES = ["hip/decode.hip", "hip/encode.hip", "hip/calibrate.hip"]
 HOST_SOURCES = ["host/sizing.cpp", "host/parser.cpp", "host/cpu_codec.cpp", "host/plan.cpp",
-                "host/capi.cpp"]
+                "host/queue.cpp", "host/capi.cpp"]
 
 
 def _compile(src: str) -> str:

@@ -475,6 +475,13 @@ int tsk_decode(const tsk_decode_args *args, void *stream) {
     return hipGetLastError();
 }
 
+int tsk_decode_occupancy(std::uint32_t kind, std::uint32_t group, std::uint32_t block, int *blocks) {
+    const DecodeFn fn = decode_function(kind, group);
+    if (fn == nullptr) return hipErrorInvalidValue;
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, reinterpret_cast<const void *>(fn),
+                                                        static_cast<int>(block), 0);
+}
+
 int tsk_fill_u32(std::uint32_t *data, std::uint64_t words, std::uint32_t value, void *stream) {
     if (words == 0) return hipSuccess;
     hipLaunchKernelGGL(fill_kernel, dim3(small_grid(words)), dim3(256), 0, static_cast<hipStream_t>(stream), data,

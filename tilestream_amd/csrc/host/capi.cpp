@@ -3,6 +3,7 @@
 // passed through without copying.
 #include "plan.h"
 
+#include <tilestream/batch.h>
 #include <tilestream/cpu.h>
 #include <tilestream/hip.h>
 #include <tilestream_c.h>
@@ -12,6 +13,7 @@
 #include <memory>
 #include <new>
 #include <string>
+#include <vector>
 
 using namespace tilestream;
 
@@ -45,6 +47,13 @@ static_assert(offsetof(ts_batch_request, tile) == offsetof(hip::BatchRequest, ti
 struct ts_plan {
     std::unique_ptr<hip::BatchPlan> plan;
 };
+
+struct ts_queue {
+    std::unique_ptr<hip::DecodeQueue> queue;
+};
+
+TS_SAME_LAYOUT(ts_launch_shape, hip::LaunchShape);
+TS_SAME_LAYOUT(ts_queue_stats, hip::QueueStats);
 
 namespace {
 
@@ -210,6 +219,68 @@ uint32_t ts_plan_set_decode_shape(ts_plan *plan, uint32_t group, uint32_t block,
     return 0;
 }
 
+uint32_t ts_queue_prepare(const ts_batch_item *items, const uint32_t *operations, uint64_t count, ts_queue **queue,
+                          void *stream) {
+    if (queue == nullptr) return code(Status::invalid_argument);
+    try {
+        std::unique_ptr<hip::DecodeQueue> fresh;
+        const Status st = hip::DecodeQueue::prepare(reinterpret_cast<const hip::BatchItem *>(items),
+                                                    reinterpret_cast<const hip::Operation *>(operations), count,
+                                                    fresh, stream_of(stream));
+        if (st != Status::success) return code(st);
+        auto *handle = new ts_queue{std::move(fresh)};
+        delete *queue;
+        *queue = handle;
+        return 0;
+    } catch (const std::bad_alloc &) {
+        return code(Status::resource_limit);
+    }
+}
+
+void ts_queue_destroy(ts_queue *queue) { delete queue; }
+uint64_t ts_queue_workspace_bytes(const ts_queue *q) { return q->queue->workspace_bytes(); }
+void ts_queue_get_stats(const ts_queue *q, ts_queue_stats *out) {
+    const hip::QueueStats s = q->queue->stats();
+    std::memcpy(out, &s, sizeof s);
+}
+void ts_queue_get_shape(const ts_queue *q, ts_launch_shape *out) {
+    const hip::LaunchShape s = q->queue->shape();
+    std::memcpy(out, &s, sizeof s);
+}
+uint32_t ts_queue_set_shape(ts_queue *q, const ts_launch_shape *shape) {
+    const uint32_t g = shape->group, b = shape->block;
+    if ((g != 1 && g != 2 && g != 4 && g != 8 && g != 16) || (b != 64 && b != 128 && b != 256 && b != 512))
+        return code(Status::invalid_argument);
+    q->queue->set_shape(as<hip::LaunchShape>(*shape));
+    return 0;
+}
+uint32_t ts_queue_run(ts_queue *q, ts_mutable_bytes ws, ts_device_result *r, uint64_t n, void *s) {
+    return code(q->queue->run(as<MutableBytes>(ws), {reinterpret_cast<DeviceResult *>(r), n}, stream_of(s)));
+}
+uint32_t ts_queue_capture(ts_queue *q, ts_mutable_bytes ws, ts_device_result *r, uint64_t n, void *s) {
+    return code(q->queue->capture(as<MutableBytes>(ws), {reinterpret_cast<DeviceResult *>(r), n}, stream_of(s)));
+}
+uint32_t ts_queue_launch(ts_queue *q, void *s) { return code(q->queue->launch(stream_of(s))); }
+uint32_t ts_queue_autotune(ts_queue *q, ts_mutable_bytes ws, ts_device_result *r, uint64_t n, void *s,
+                           const ts_launch_shape *candidates, uint64_t count, uint32_t repeats, float *ms) {
+    try {
+        std::vector<hip::LaunchShape> c(count);
+        for (uint64_t i = 0; i < count; ++i) {
+            c[i] = as<hip::LaunchShape>(candidates[i]);
+            const uint32_t g = c[i].group, b = c[i].block;
+            if ((g != 1 && g != 2 && g != 4 && g != 8 && g != 16) || (b != 64 && b != 128 && b != 256 && b != 512))
+                return code(Status::invalid_argument);
+        }
+        std::vector<float> times;
+        const Status st = q->queue->autotune(as<MutableBytes>(ws), {reinterpret_cast<DeviceResult *>(r), n},
+                                             stream_of(s), c, repeats, times);
+        for (uint64_t i = 0; i < count && i < times.size(); ++i) ms[i] = times[i];
+        return code(st);
+    } catch (const std::bad_alloc &) {
+        return code(Status::resource_limit);
+    }
+}
+
 int ts_hip_malloc(uint64_t bytes, void **data) { return hipMalloc(data, bytes); }
 int ts_hip_free(void *data) { return hipFree(data); }
 int ts_hip_host_malloc(uint64_t bytes, void **data) { return hipHostMalloc(data, bytes, hipHostMallocMapped); }
@@ -269,6 +340,8 @@ uint64_t ts_abi_sizeof(const char *name) {
     TS_SIZE(ts_compressed_input)
     TS_SIZE(ts_batch_item)
     TS_SIZE(ts_batch_request)
+    TS_SIZE(ts_launch_shape)
+    TS_SIZE(ts_queue_stats)
 #undef TS_SIZE
     return 0;
 }
