@@ -1,0 +1,133 @@
+"""Calibration r = RN(RN(x - offset) * gain): standalone and fused with decode.
+
+Tolerance: FP32 results equal the host binary32 computation bit for bit
+(compiled with -ffp-contract=off), NaN compared by class; against the FP64
+reference the error stays below 1.5 ulp for normal results and below 2^-149
+absolute for subnormal ones (reference docs/numerical-analysis.md "Error bounds").
+"""
+import numpy as np
+import pytest
+
+import tilestream_amd as ts
+from tilestream_amd import api, generators as g
+from tilestream_amd._native import BatchItem
+
+from helpers import DeviceOutput, Job
+
+pytestmark = pytest.mark.gpu
+
+SPECIAL = np.array([0x00000000, 0x80000000, 0x7F800000, 0xFF800000, 0x7FC00000, 0xFFC00001, 0x7F800001,
+                    0x00000001, 0x807FFFFF, 0x00800000, 0x3F800000, 0xBF800000, 0x7F7FFFFF, 0xFF7FFFFF,
+                    0x3F800001, 0x4B800000, 0x33800000, 0x00400000, 0x80000001, 0x42C80000, 0xC2C80000],
+                   np.uint32)
+
+
+def same_or_both_nan(a, b):
+    fa, fb = a.view(np.float32), b.view(np.float32)
+    both_nan = np.isnan(fa) & np.isnan(fb)
+    return bool(np.all((a == b) | both_nan)), int(np.sum(~((a == b) | both_nan)))
+
+
+def calibrate_on_gpu(samples, offset, gain):
+    h, w = samples.shape
+    ptype = api.pixel_type_of(samples.dtype)
+    src = api.DeviceMemory.from_host(samples)
+    out = DeviceOutput(ts.PixelType.float32, w, h)
+    job = Job()
+    item = BatchItem()
+    item.input = api.device_view(src, ptype, w, h, samples.strides[0], const=True)
+    item.output = out.view()
+    item.calibration.offset = job._param(offset, w, h, 0.0)
+    item.calibration.gain = job._param(gain, w, h, 1.0)
+    st, plan = api.prepare_batch(ts.Operation.calibrate, [item])
+    assert st == ts.Status.success
+    assert plan.workspace_bytes() == 0
+    results = api.DeviceMemory(16)
+    assert api.calibrate_async(plan, None, results, 1) == ts.Status.success
+    assert api.synchronize() == ts.Status.success
+    status, written = api.read_results(results, 1)
+    assert status[0] == 0 and written[0] == w * h * 4
+    return out.read()
+
+
+def test_standalone_matches_host_binary32(orc):
+    px = g.detector_u16(333, 97, 3)
+    dark = g.dark_u16(333, 97, 4).astype(np.float32)
+    gain = g.gain_f32(333, 97, 5)
+    got = calibrate_on_gpu(px, dark, gain)
+    want = orc.calibrate(px, dark, gain).view(np.uint32)
+    assert np.array_equal(got, want)
+    f = g.field_f32(333, 97, 6)
+    got = calibrate_on_gpu(f, 100.0, 1.05)
+    ok, bad = same_or_both_nan(got, orc.calibrate(f, 100.0, 1.05).view(np.uint32))
+    assert ok, bad
+
+
+def test_every_uint16_value():
+    px = np.arange(65536, dtype=np.uint32).astype(np.uint16).reshape(256, 256)
+    got = calibrate_on_gpu(px, 100.25, 1.5).view(np.float32)
+    want = ((px.astype(np.float32) - np.float32(100.25)).astype(np.float32) * np.float32(1.5)).astype(np.float32)
+    assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
+
+
+def test_special_word_grid(orc):
+    """21 x 21 x 21 grid of special sample, offset and gain words."""
+    n = len(SPECIAL)
+    x = np.repeat(SPECIAL, n * n).reshape(n, n * n)
+    o = np.tile(np.repeat(SPECIAL, n), n).reshape(n, n * n).view(np.float32)
+    gn = np.tile(SPECIAL, n * n).reshape(n, n * n).view(np.float32)
+    got = calibrate_on_gpu(x, o, gn)
+    ok, bad = same_or_both_nan(got, orc.calibrate(x, o, gn).view(np.uint32))
+    assert ok, bad
+
+
+def _fp64_errors(samples, offset, gain, result, u16):
+    x = samples.astype(np.float64) if u16 else samples.view(np.float32).astype(np.float64)
+    ref = (x - np.broadcast_to(np.float64(offset), x.shape)) * np.broadcast_to(np.float64(gain), x.shape)
+    r = result.view(np.float32).astype(np.float64)
+    finite = np.isfinite(ref) & np.isfinite(r) & (np.abs(ref) < 2.0 ** 128 - 2.0 ** 103)
+    e = np.frexp(np.where(ref == 0, 1.0, np.abs(ref)))[1] - 1
+    ulp = np.where(np.abs(ref) >= 2.0 ** -126, np.ldexp(1.0, e - 23), 2.0 ** -149)
+    err = np.abs(r - ref)[finite] / ulp[finite]
+    normal = (np.abs(ref) >= 2.0 ** -126)[finite]
+    return err[normal], np.abs(r - ref)[finite][~normal]
+
+
+@pytest.mark.parametrize("kind", ["u16", "f32"])
+def test_fused_equals_separate_and_bound(orc, kind):
+    w, h = 700, 389
+    px = g.detector_u16(w, h, 31) if kind == "u16" else g.field_f32(w, h, 31)
+    data = orc.compress(px, (64, 64))
+    roi = (37, 21, 500, 300)
+    x0, y0, rw, rh = roi
+    dark = g.dark_u16(rw, rh, 32).astype(np.float32)
+    gain = g.gain_f32(rw, rh, 33)
+    job = Job()
+    job.add_decode(data, roi=roi, calibrated=True, offset=dark, gain=gain)
+    status, written, outs = job.run(ts.Operation.decode_calibrate)
+    assert status[0] == 0 and written[0] == rw * rh * 4
+    fused = outs[0]
+    crop = np.ascontiguousarray(px[y0:y0 + rh, x0:x0 + rw])
+    separate = calibrate_on_gpu(crop, dark, gain)
+    assert np.array_equal(fused, separate)  # same device operations, identical words, NaNs included
+    ok, bad = same_or_both_nan(fused, orc.decode_roi_calibrate(data, roi, dark, gain).view(np.uint32))
+    assert ok, bad
+    normal, sub = _fp64_errors(crop, dark, gain, fused, kind == "u16")
+    assert normal.max() < 1.5 + 2.0 ** -20
+    assert sub.size == 0 or sub.max() < 2.0 ** -149 * (1 + 2.0 ** -20)
+
+
+@pytest.mark.parametrize("tile", [(128, 32), (64, 64), (256, 16)])
+def test_fused_scalar_region(orc, tile):
+    px = g.detector_u16(1000, 600, 51)
+    data = orc.compress(px, tile)
+    rng = np.random.default_rng(52)
+    for _ in range(4):
+        rw, rh = int(rng.integers(1, 1000)), int(rng.integers(1, 600))
+        x, y = int(rng.integers(0, 1000 - rw + 1)), int(rng.integers(0, 600 - rh + 1))
+        job = Job()
+        job.add_decode(data, roi=(x, y, rw, rh), calibrated=True, offset=100.0, gain=1.05)
+        status, _, outs = job.run(ts.Operation.decode_calibrate)
+        assert status[0] == 0
+        want = orc.decode_roi_calibrate(data, (x, y, rw, rh), 100.0, 1.05).view(np.uint32)
+        assert np.array_equal(outs[0], want)

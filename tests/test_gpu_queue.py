@@ -1,0 +1,135 @@
+"""Batched persistent decode (DecodeQueue): a mixed batch of pixel types, tile
+shapes, regions and fused calibration in one launch, graph capture and replay,
+the autotuner, and each result identical alone and in the mix."""
+import numpy as np
+import pytest
+
+import tilestream_amd as ts
+from tilestream_amd import api, generators as g
+from tilestream_amd.workloads import ResidentBatch
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def batch():
+    layout = g.batch_layout(24, 77, lo=64, hi=1200, step=16)
+    return ResidentBatch.build(layout, 77)
+
+
+def _run_queue(batch, ops, rois, shape=None, stream=None, graph_replays=0, cal=None):
+    items = []
+    for i, (op, roi) in enumerate(zip(ops, rois)):
+        it = batch.item(i, roi)
+        if op == ts.Operation.decode_calibrate:
+            e = batch.entries[i]
+            w, h = roi[2], roi[3]
+            it.output = api.device_view(cal[i], ts.PixelType.float32, w, h, w * 4)
+            it.calibration.offset = api.scalar(100.0)
+            it.calibration.gain = api.scalar(1.05)
+        items.append(it)
+    st, q = api.DecodeQueue.prepare(items, ops, stream)
+    assert st == ts.Status.success, st
+    if shape:
+        assert q.set_shape(*shape) == ts.Status.success
+    ws = api.DeviceMemory(max(256, q.workspace_bytes()))
+    results = api.DeviceMemory(16 * len(items))
+    if graph_replays:
+        s = stream or api.Stream()
+        assert q.capture(ws, results, s) == ts.Status.success
+        for _ in range(graph_replays):
+            batch.clear_outputs()
+            assert q.launch(s) == ts.Status.success
+            s.synchronize()
+            yield api.read_results(results, len(items))
+        return
+    assert q.run(ws, results, stream) == ts.Status.success
+    api.device_synchronize()
+    yield api.read_results(results, len(items))
+
+
+def test_whole_batch_matches_pixels(batch):
+    n = len(batch.entries)
+    for shape in [(8, 256, 0, 1), (4, 128, 0, 4), (16, 512, 0, 2), (8, 256, 0, 0), (2, 64, 37, 1)]:
+        batch.clear_outputs()
+        ops = [ts.Operation.decompress] * n
+        rois = [(0, 0, e.width, e.height) for e in batch.entries]
+        (status, written), = list(_run_queue(batch, ops, rois, shape))
+        assert (status == 0).all()
+        for i, e in enumerate(batch.entries):
+            assert written[i] == e.raw_bytes
+            assert np.array_equal(batch.read_output(i), e.pixels), (shape, i)
+
+
+def test_graph_replays_are_identical(batch):
+    n = len(batch.entries)
+    ops = [ts.Operation.decompress] * n
+    rois = [(0, 0, e.width, e.height) for e in batch.entries]
+    for status, written in _run_queue(batch, ops, rois, graph_replays=4):
+        assert (status == 0).all()
+        for i, e in enumerate(batch.entries):
+            assert np.array_equal(batch.read_output(i), e.pixels)
+
+
+def test_mixed_regions_and_calibration_alone_and_in_mix(batch, orc):
+    rng = np.random.default_rng(78)
+    n = len(batch.entries)
+    ops, rois, cal = [], [], {}
+    for i, e in enumerate(batch.entries):
+        w, h = int(rng.integers(1, e.width + 1)), int(rng.integers(1, e.height + 1))
+        x, y = int(rng.integers(0, e.width - w + 1)), int(rng.integers(0, e.height - h + 1))
+        rois.append((x, y, w, h))
+        op = [ts.Operation.decompress, ts.Operation.decode_roi, ts.Operation.decode_calibrate][i % 3]
+        if op == ts.Operation.decompress:
+            rois[-1] = (0, 0, e.width, e.height)
+        if op == ts.Operation.decode_calibrate:
+            cal[i] = api.DeviceMemory(w * h * 4)
+        ops.append(op)
+    # in the mix
+    batch.clear_outputs()
+    (status, _), = list(_run_queue(batch, ops, rois, cal=cal))
+    assert (status == 0).all()
+    mixed = {}
+    for i, e in enumerate(batch.entries):
+        x, y, w, h = rois[i]
+        if ops[i] == ts.Operation.decode_calibrate:
+            got = cal[i].download().view(np.uint32).reshape(h, w)
+            want = orc.decode_roi_calibrate(e.archive, rois[i], 100.0, 1.05).view(np.uint32)
+            fg, fw = got.view(np.float32), want.view(np.float32)
+            assert np.all((got == want) | (np.isnan(fg) & np.isnan(fw)))
+        else:
+            raw = batch.out.download(w * h * e.sample, e.out_offset)
+            got = raw.view(np.uint16 if e.sample == 2 else np.uint32).reshape(h, w)
+            assert np.array_equal(got, e.pixels[y:y + h, x:x + w])
+        mixed[i] = got.copy()
+    # alone: each item in a queue of its own gives the same words
+    for i in range(0, n, 5):
+        batch.clear_outputs()
+        x, y, w, h = rois[i]
+        sub = ResidentBatch([batch.entries[i]], batch.src, batch.out)
+        (status, _), = list(_run_queue(sub, [ops[i]], [rois[i]], cal={0: cal[i]} if i in cal else None))
+        assert status[0] == 0
+        e = batch.entries[i]
+        if ops[i] == ts.Operation.decode_calibrate:
+            alone = cal[i].download().view(np.uint32).reshape(h, w)
+        else:
+            raw = batch.out.download(w * h * e.sample, e.out_offset)
+            alone = raw.view(np.uint16 if e.sample == 2 else np.uint32).reshape(h, w)
+        assert np.array_equal(alone, mixed[i])
+
+
+def test_autotune_picks_a_candidate(batch):
+    n = len(batch.entries)
+    items = [batch.item(i) for i in range(n)]
+    st, q = api.DecodeQueue.prepare(items, [ts.Operation.decompress] * n)
+    assert st == ts.Status.success
+    ws = api.DeviceMemory(256)
+    results = api.DeviceMemory(16 * n)
+    s = api.Stream()
+    cands = [(4, 256, 0, 1), (8, 256, 0, 1), (16, 128, 0, 2)]
+    st, ms = q.autotune(ws, results, s, cands, 2)
+    assert st == ts.Status.success and all(t > 0 for t in ms)
+    best = q.shape
+    assert (best.group, best.block, best.grid, best.chunk) == cands[int(np.argmin(ms))]
+    st = q.stats()
+    assert st.images == n and st.decoded_bytes == batch.decoded_bytes
