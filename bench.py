@@ -93,6 +93,10 @@ def main() -> int:
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--no-verify", action="store_true")
     p.add_argument("--tune-repeats", type=int, default=3)
+    p.add_argument("--profile", action="store_true",
+                   help="only the batched launches (for rocprofv3): no autotune, per-image path, "
+                        "CPU baseline or verification")
+    p.add_argument("--shape", default="8,256,1", help="group,block,chunk used with --profile")
     args = p.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (this codec has no "
@@ -123,17 +127,25 @@ def main() -> int:
     qws = api.DeviceMemory(max(queue.workspace_bytes(), 256))
     candidates = [(g, b, 0, c) for g in (4, 8, 16) for b in (128, 256, 512) for c in (1, 4)]
     candidates += [(8, 256, 0, 0), (16, 256, 0, 0)]
-    st, times = queue.autotune(qws, results, stream, candidates, args.tune_repeats)
-    assert st == api.Status.success, st
+    if args.profile:
+        grp, blk, chk = (int(v) for v in args.shape.split(","))
+        assert queue.set_shape(grp, blk, 0, chk) == api.Status.success
+    else:
+        st, times = queue.autotune(qws, results, stream, candidates, args.tune_repeats)
+        assert st == api.Status.success, st
+        log("autotune " + " ".join(f"{c[0]}/{c[1]}/{c[3]}:{t:.3f}" for c, t in zip(candidates, times)))
     best = queue.shape
-    log(f"autotune best group={best.group} block={best.block} chunk={best.chunk} "
-        f"({min(times):.3f} ms); worst {max(times):.3f} ms")
+    log(f"shape group={best.group} block={best.block} chunk={best.chunk}")
     assert queue.capture(qws, results, stream) == api.Status.success
     launch = lambda: queue.launch(stream)  # noqa: E731
     for _ in range(args.warmup):
         launch()
     ms_batched = timed(stream, launch, args.steps)
     stats = queue.stats()
+    if args.profile:
+        log(f"batched {ms_batched:.4f} ms per launch, "
+            f"{batch.decoded_bytes / (ms_batched * 1e-3) / 1e9:.1f} GB/s decoded")
+        return 0
 
     # ---- plain per-image path ------------------------------------------------------
     plans, ws_bytes = [], 0

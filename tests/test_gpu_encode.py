@@ -132,15 +132,17 @@ def _trusted_decode(enc, index, roi=None):
 
 @pytest.mark.parametrize("tile", SHAPES)
 def test_trusted_encoder_output_decodes(tile):
-    imgs = [g.detector_u16(517, 301, 64), g.field_f32(300, 250, 65)]
-    enc = Encoded(imgs, tile)
-    enc.tile = tile
-    enc.run()
-    for i, px in enumerate(imgs):
-        st, n, out = _trusted_decode(enc, i)
-        assert st == 0 and n == px.nbytes and np.array_equal(out, px)
-        st, n, out = _trusted_decode(enc, i, (13, 7, 200, 100))
-        assert st == 0 and np.array_equal(out, px[7:107, 13:213])
+    for imgs in ([g.detector_u16(517, 301, 64), g.detector_u16(64, 70, 67)],
+                 [g.field_f32(300, 250, 65), g.field_f32(129, 33, 68)]):
+        enc = Encoded(imgs, tile)
+        enc.tile = tile
+        enc.run()
+        for i, px in enumerate(imgs):
+            st, n, out = _trusted_decode(enc, i)
+            assert st == 0 and n == px.nbytes and np.array_equal(out, px)
+            rw, rh = min(60, px.shape[1] - 3), min(25, px.shape[0] - 2)
+            st, n, out = _trusted_decode(enc, i, (3, 2, rw, rh))
+            assert st == 0 and np.array_equal(out, px[2:2 + rh, 3:3 + rw])
 
 
 def test_trusted_path_detects_corruption_and_producer_failure():
