@@ -96,7 +96,9 @@ def main() -> int:
     p.add_argument("--profile", action="store_true",
                    help="only the batched launches (for rocprofv3): no autotune, per-image path, "
                         "CPU baseline or verification")
-    p.add_argument("--shape", default="8,256,1", help="group,block,chunk used with --profile")
+    p.add_argument("--shape", default="8,256,8,2", help="group,block,chunk,depth used with --profile")
+    p.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                   help="only run a plain copy kernel of this many MiB (scales HBM byte counters)")
     args = p.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (this codec has no "
@@ -109,6 +111,16 @@ def main() -> int:
     if api.device_count() < 1:
         print("no GPU visible", file=sys.stderr)
         return 1
+    if args.copy_probe:
+        from tilestream_amd import _native as N
+        nbytes = args.copy_probe << 20
+        a, b, s = api.DeviceMemory(nbytes), api.DeviceMemory(nbytes), api.Stream()
+        a.fill(1)
+        for _ in range(max(1, args.steps)):
+            assert N.lib().tsk_copy_blocks(b.ptr, a.ptr, nbytes // 16, s.handle) == 0
+        s.synchronize()
+        log(f"copy probe: {args.steps} copies of {nbytes} bytes (read {nbytes}, write {nbytes} each)")
+        return 0
     threads = max(1, min(16, os.cpu_count() or 1))
     t0 = time.perf_counter()
     layout = generators.batch_layout(args.images, args.seed)
@@ -125,17 +137,18 @@ def main() -> int:
     st, queue = api.DecodeQueue.prepare(items, [api.Operation.decompress] * n, stream)
     assert st == api.Status.success, st
     qws = api.DeviceMemory(max(queue.workspace_bytes(), 256))
-    candidates = [(g, b, 0, c) for g in (4, 8, 16) for b in (128, 256, 512) for c in (1, 4)]
-    candidates += [(8, 256, 0, 0), (16, 256, 0, 0)]
+    candidates = [(g, b, 0, c, d) for g in (4, 8, 16) for (b, d) in ((128, 1), (256, 1), (256, 2), (512, 2))
+                  for c in (4, 16)]
+    candidates += [(8, 256, 0, 0, 2), (4, 256, 0, 0, 2)]
     if args.profile:
-        grp, blk, chk = (int(v) for v in args.shape.split(","))
-        assert queue.set_shape(grp, blk, 0, chk) == api.Status.success
+        grp, blk, chk, dep = (int(v) for v in args.shape.split(","))
+        assert queue.set_shape(grp, blk, 0, chk, dep) == api.Status.success
     else:
         st, times = queue.autotune(qws, results, stream, candidates, args.tune_repeats)
         assert st == api.Status.success, st
-        log("autotune " + " ".join(f"{c[0]}/{c[1]}/{c[3]}:{t:.3f}" for c, t in zip(candidates, times)))
+        log("autotune " + " ".join(f"{c[0]}/{c[1]}/{c[3]}/{c[4]}:{t:.3f}" for c, t in zip(candidates, times)))
     best = queue.shape
-    log(f"shape group={best.group} block={best.block} chunk={best.chunk}")
+    log(f"shape group={best.group} block={best.block} chunk={best.chunk} depth={best.depth}")
     assert queue.capture(qws, results, stream) == api.Status.success
     launch = lambda: queue.launch(stream)  # noqa: E731
     for _ in range(args.warmup):
@@ -216,7 +229,7 @@ def main() -> int:
             "note": "payload bytes of every tile read + decoded output bytes written, over the "
                     "graph launch time (results init, counter reset, decode, results completion)",
         },
-        "autotune": {"group": best.group, "block": best.block, "chunk": best.chunk,
+        "autotune": {"group": best.group, "block": best.block, "chunk": best.chunk, "depth": best.depth,
                      "candidates": len(candidates)},
         "cpu_baseline": cpu,
         "verified_bit_exact": verified,

@@ -116,15 +116,16 @@ uint32_t ts_buffer_release(void *data, void *stream);
 uint32_t ts_synchronize(void *stream);
 
 /* ---- extension: launch shape of the decode kernel ---------------------- */
-/* group: columns per lane (1,2,4,8,16); block: 64..512; grid: 0 = one workgroup per tile. */
-uint32_t ts_plan_set_decode_shape(ts_plan *plan, uint32_t group, uint32_t block, uint32_t grid);
+/* group: columns per lane (1,2,4,8,16); block: 64..512; grid: 0 = one workgroup per tile;
+ * depth: 1 staged, 2 next tile prefetched.  invalid_argument for a combination without a kernel. */
+uint32_t ts_plan_set_decode_shape(ts_plan *plan, uint32_t group, uint32_t block, uint32_t grid, uint32_t depth);
 
 /* ---- extension: batched persistent decode (tilestream/batch.h) ---------
  * No counterpart in the reference.  Items may mix pixel types, tile shapes,
  * regions and fused calibration; operations[i] is 1 decompress, 2 decode_roi
  * or 4 decode_calibrate. */
 typedef struct ts_queue ts_queue;
-typedef struct { uint32_t group, block, grid, chunk; } ts_launch_shape;
+typedef struct { uint32_t group, block, grid, chunk, depth; } ts_launch_shape;
 typedef struct {
     uint64_t images, tasks, decoded_bytes, output_bytes, payload_bytes, map_bytes;
 } ts_queue_stats;

@@ -60,30 +60,35 @@ typedef struct {
     float offset, gain;          /* scalar parameters */
 } tsk_decode_image;
 
-/* One tile to decode.  On the trusted path mode/offset/words are read from the
- * device-resident index instead. */
+/* One tile to decode.  On the trusted path the payload address, mode and
+ * length are read from the device-resident index instead (payload = 0). */
 typedef struct {
-    uint64_t offset;             /* payload byte offset within image.src */
-    uint64_t tile_index;         /* tile number in the archive */
+    const uint8_t *payload;      /* absolute device address of the tile payload */
+    uint32_t column, row;        /* tile position in the archive's tile grid */
     uint32_t image;              /* index into the image table */
     uint32_t rows;               /* first entry of the row table, TSK_NO_ROWS if none */
     uint32_t words;              /* payload length / 4 */
-    uint32_t mode;               /* 0 RAW, 1 CONSTANT, 2 PREDICT_BITPACK */
+    uint32_t shape;              /* mode | actual height << 8 | actual width << 16 */
 } tsk_decode_task;
+
+#define TSK_TASK_SHAPE(mode, height, width) ((uint32_t)(mode) | ((uint32_t)(height) << 8) | ((uint32_t)(width) << 16))
 
 typedef struct {
     const tsk_decode_image *images;
     const tsk_decode_task *tasks;
     const uint16_t *rows;        /* word offset of each predictive row in its payload */
     void *results;               /* DeviceResult[image_count] */
-    uint32_t *queue;             /* work counter for the dynamic queue, NULL = static stride */
+    uint64_t *queue;             /* work counter for the dynamic queue, NULL = static stride */
     uint64_t task_count;
+    uint64_t row_count;          /* entries in the row table */
     uint32_t image_count;
     uint32_t kind;               /* TSK_KIND_* */
     uint32_t group;              /* columns per lane: 1, 2, 4, 8 or 16 */
     uint32_t block;              /* threads per workgroup: 64, 128, 256 or 512 */
     uint32_t grid;               /* workgroups; tiles are strided or drawn from the queue */
     uint32_t chunk;              /* tiles taken per queue draw */
+    uint32_t depth;              /* tiles a workgroup holds at once: 1 staged, 2 = + next prefetched */
+    uint32_t reserved;
 } tsk_decode_args;
 
 /* Per-image descriptor of an encode job. */
@@ -147,7 +152,9 @@ int tsk_results_complete(const tsk_decode_image *images, uint32_t count, void *r
                          void *stream);
 int tsk_decode(const tsk_decode_args *args, void *stream);
 /* Resident workgroups per CU of a decode kernel variant (for persistent grids). */
-int tsk_decode_occupancy(uint32_t kind, uint32_t group, uint32_t block, int *blocks);
+int tsk_decode_occupancy(uint32_t kind, uint32_t group, uint32_t block, uint32_t depth, int *blocks);
+/* Whether a (group, block, depth) combination has a compiled kernel. */
+int tsk_decode_supported(uint32_t group, uint32_t block, uint32_t depth);
 int tsk_encode_analyze(const tsk_encode_args *args, void *stream);
 int tsk_encode_scan(const tsk_encode_args *args, void *stream);
 int tsk_encode_pack(const tsk_encode_args *args, void *stream);
@@ -156,6 +163,9 @@ int tsk_calibrate(const tsk_calibrate_args *args, void *stream);
 /* Loads every kernel of an operation so that no execution loads code.
  * op: 0 compress, 1 decode, 2 calibrate, 3 fused decode. */
 int tsk_preload(uint32_t op);
+/* Copies `blocks` 16-byte blocks with a plain streaming kernel (the known-size
+ * reference for scaling hardware byte counters). */
+int tsk_copy_blocks(void *dst, const void *src, uint64_t blocks, void *stream);
 /* Zero-fills `words` 32-bit words (a kernel, safe under graph capture). */
 int tsk_fill_u32(uint32_t *data, uint64_t words, uint32_t value, void *stream);
 

@@ -91,7 +91,7 @@ class Job:
         st, plan = api.prepare_batch(operation, self.items)
         assert st == ts.Status.success, st
         if shape is not None:
-            assert plan.set_decode_shape(*shape) == ts.Status.success
+            assert plan.set_decode_shape(*shape) == ts.Status.success, shape
         ws = api.DeviceMemory(plan.workspace_bytes() + workspace_extra) if plan.workspace_bytes() else None
         results = api.DeviceMemory(16 * len(self.items))
         results.fill(0x5A)

@@ -32,14 +32,19 @@ def test_whole_image_shapes(orc, tile, kind):
     assert np.array_equal(out, px)
 
 
-@pytest.mark.parametrize("group", [1, 2, 4, 8, 16])
-@pytest.mark.parametrize("block", [64, 256, 512])
-def test_lane_group_widths(orc, group, block):
+SHAPE_GRID = [(g_, b_, d_) for g_ in (1, 2, 4, 8, 16) for b_ in (64, 128, 256, 512) for d_ in (1, 2)]
+
+
+@pytest.mark.parametrize("group,block,depth", SHAPE_GRID)
+def test_lane_group_widths(orc, group, block, depth):
+    if not ts.api.decode_shape_supported(group, block, depth):
+        pytest.skip("no kernel compiled for this shape")
     for kind, tile in [("u16", (128, 32)), ("f32", (64, 64)), ("u16", (256, 16))]:
         px = g.detector_u16(300, 200, 9) if kind == "u16" else g.field_f32(300, 200, 9)
         data = orc.compress(px, tile)
-        out, _ = decode_on_gpu(data, shape=(group, block, 0))
-        assert np.array_equal(out, px), (kind, tile, group, block)
+        for grid in (0, 3):  # one workgroup per tile, and 3 workgroups looping over all tiles
+            out, _ = decode_on_gpu(data, shape=(group, block, grid, depth))
+            assert np.array_equal(out, px), (kind, tile, group, block, depth, grid)
 
 
 def test_every_width_and_partial_rows(orc):
@@ -55,7 +60,7 @@ def test_every_width_and_partial_rows(orc):
             px = np.stack(rows)
             for tile in SHAPES:
                 data = orc.compress(px, tile)
-                out, _ = decode_on_gpu(data, shape=(8, 256, 0))
+                out, _ = decode_on_gpu(data, shape=(8, 256, 0, 1))
                 assert np.array_equal(out, px), (ptype, w, tile)
 
 

@@ -50,7 +50,8 @@ def _run_queue(batch, ops, rois, shape=None, stream=None, graph_replays=0, cal=N
 
 def test_whole_batch_matches_pixels(batch):
     n = len(batch.entries)
-    for shape in [(8, 256, 0, 1), (4, 128, 0, 4), (16, 512, 0, 2), (8, 256, 0, 0), (2, 64, 37, 1)]:
+    for shape in [(8, 256, 0, 1, 2), (4, 128, 0, 4, 1), (16, 512, 0, 2, 2), (8, 256, 0, 0, 2),
+                  (2, 256, 37, 1, 1), (4, 512, 5, 3, 2), (8, 64, 0, 16, 1)]:
         batch.clear_outputs()
         ops = [ts.Operation.decompress] * n
         rois = [(0, 0, e.width, e.height) for e in batch.entries]
@@ -126,10 +127,10 @@ def test_autotune_picks_a_candidate(batch):
     ws = api.DeviceMemory(256)
     results = api.DeviceMemory(16 * n)
     s = api.Stream()
-    cands = [(4, 256, 0, 1), (8, 256, 0, 1), (16, 128, 0, 2)]
+    cands = [(4, 256, 0, 1, 2), (8, 256, 0, 8, 1), (16, 128, 0, 2, 1)]
     st, ms = q.autotune(ws, results, s, cands, 2)
     assert st == ts.Status.success and all(t > 0 for t in ms)
     best = q.shape
-    assert (best.group, best.block, best.grid, best.chunk) == cands[int(np.argmin(ms))]
+    assert (best.group, best.block, best.grid, best.chunk, best.depth) == cands[int(np.argmin(ms))]
     st = q.stats()
     assert st.images == n and st.decoded_bytes == batch.decoded_bytes

@@ -120,7 +120,8 @@ class BatchRequest(C.Structure):
 
 
 class LaunchShape(C.Structure):
-    _fields_ = [("group", C.c_uint32), ("block", C.c_uint32), ("grid", C.c_uint32), ("chunk", C.c_uint32)]
+    _fields_ = [("group", C.c_uint32), ("block", C.c_uint32), ("grid", C.c_uint32), ("chunk", C.c_uint32),
+                ("depth", C.c_uint32)]
 
 
 class QueueStats(C.Structure):
@@ -168,7 +169,7 @@ PROTOTYPES = {
     "ts_buffer_allocate": (_u32, [_u64, _vp, _P(_vp)]),
     "ts_buffer_release": (_u32, [_vp, _vp]),
     "ts_synchronize": (_u32, [_vp]),
-    "ts_plan_set_decode_shape": (_u32, [_vp, _u32, _u32, _u32]),
+    "ts_plan_set_decode_shape": (_u32, [_vp, _u32, _u32, _u32, _u32]),
     "ts_queue_prepare": (_u32, [_P(BatchItem), _P(_u32), _u64, _P(_vp), _vp]),
     "ts_queue_destroy": (None, [_vp]),
     "ts_queue_workspace_bytes": (_u64, [_vp]),
@@ -200,6 +201,8 @@ PROTOTYPES = {
     "ts_hip_event_synchronize": (_int, [_vp]),
     "ts_hip_event_elapsed_ms": (_int, [_vp, _vp, _P(C.c_float)]),
     "ts_abi_sizeof": (_u64, [C.c_char_p]),
+    "tsk_copy_blocks": (_int, [_vp, _vp, _u64, _vp]),
+    "tsk_decode_supported": (_int, [_u32, _u32, _u32]),
 }
 
 _lib = None

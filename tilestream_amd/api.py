@@ -289,6 +289,11 @@ def _stream(stream):
     return stream.handle if isinstance(stream, Stream) else stream
 
 
+def decode_shape_supported(group: int, block: int, depth: int) -> bool:
+    """Whether the library has a decode kernel for this (group, block, depth)."""
+    return bool(N.lib().tsk_decode_supported(group, block, depth))
+
+
 def device_count() -> int:
     n = C.c_int(0)
     err = N.lib().ts_hip_device_count(C.byref(n))
@@ -327,8 +332,8 @@ class Plan:
         assert st == Status.success
         return n
 
-    def set_decode_shape(self, group=8, block=256, grid=0) -> Status:
-        return Status(N.lib().ts_plan_set_decode_shape(self.handle, group, block, grid))
+    def set_decode_shape(self, group=8, block=256, grid=0, depth=1) -> Status:
+        return Status(N.lib().ts_plan_set_decode_shape(self.handle, group, block, grid, depth))
 
     def close(self) -> None:
         if self.handle:
@@ -436,8 +441,8 @@ class DecodeQueue:
         N.lib().ts_queue_get_shape(self.handle, C.byref(s))
         return s
 
-    def set_shape(self, group=8, block=256, grid=0, chunk=1) -> Status:
-        return Status(N.lib().ts_queue_set_shape(self.handle, C.byref(LaunchShape(group, block, grid, chunk))))
+    def set_shape(self, group=8, block=256, grid=0, chunk=8, depth=2) -> Status:
+        return Status(N.lib().ts_queue_set_shape(self.handle, C.byref(LaunchShape(group, block, grid, chunk, depth))))
 
     def _ws(self, workspace):
         return MutableBytes(workspace.ptr, workspace.size) if workspace is not None else MutableBytes(None, 0)

@@ -28,7 +28,8 @@ COMMON = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-fno-fast-math",
 HIP_FLAGS = ["-x", "hip", "--offload-arch=" + ARCH, "-fno-gpu-rdc",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 
-HIP_SOURCES = ["hip/decode.hip", "hip/encode.hip", "hip/calibrate.hip"]
+HIP_SOURCES = ["hip/decode_mixed.hip", "hip/decode_u16.hip", "hip/decode_f32.hip", "hip/decode_u16_cal.hip",
+               "hip/decode_f32_cal.hip", "hip/decode.hip", "hip/encode.hip", "hip/calibrate.hip"]
 HOST_SOURCES = ["host/sizing.cpp", "host/parser.cpp", "host/cpu_codec.cpp", "host/plan.cpp",
                 "host/queue.cpp", "host/capi.cpp"]
 
@@ -52,7 +53,7 @@ def build(verbose: bool = True) -> None:
     os.makedirs(OBJ, exist_ok=True)
     os.makedirs(OUT, exist_ok=True)
     sources = HIP_SOURCES + HOST_SOURCES
-    workers = min(len(sources), max(1, min(os.cpu_count() or 4, 8)))
+    workers = min(len(sources), max(1, min(os.cpu_count() or 4, 16)))
     with cf.ThreadPoolExecutor(workers) as pool:
         objs = list(pool.map(_compile, sources))
     lib = os.path.join(OUT, "libtilestream_amd.so")

@@ -1,0 +1,4 @@
+// Decode kernels of kind TSK_KIND_F32 (see decode_core.h).
+#include "decode_core.h"
+
+TSK_DEFINE_DECODE_TABLE(tsk_decode_table_f32, TSK_KIND_F32)

@@ -209,13 +209,11 @@ uint32_t ts_buffer_release(void *data, void *stream) {
 
 uint32_t ts_synchronize(void *stream) { return code(hip::synchronize(stream_of(stream))); }
 
-uint32_t ts_plan_set_decode_shape(ts_plan *plan, uint32_t group, uint32_t block, uint32_t grid) {
+uint32_t ts_plan_set_decode_shape(ts_plan *plan, uint32_t group, uint32_t block, uint32_t grid, uint32_t depth) {
     if (plan == nullptr || plan->plan == nullptr) return code(Status::invalid_argument);
-    if ((group != 1 && group != 2 && group != 4 && group != 8 && group != 16) ||
-        (block != 64 && block != 128 && block != 256 && block != 512))
-        return code(Status::invalid_argument);
+    if (!tsk_decode_supported(group, block, depth)) return code(Status::invalid_argument);
     auto *state = const_cast<hip::BatchPlan::Impl *>(plan->plan->impl());
-    state->shape = {group, block, grid};
+    state->shape = {group, block, grid, depth};
     return 0;
 }
 
@@ -248,9 +246,7 @@ void ts_queue_get_shape(const ts_queue *q, ts_launch_shape *out) {
     std::memcpy(out, &s, sizeof s);
 }
 uint32_t ts_queue_set_shape(ts_queue *q, const ts_launch_shape *shape) {
-    const uint32_t g = shape->group, b = shape->block;
-    if ((g != 1 && g != 2 && g != 4 && g != 8 && g != 16) || (b != 64 && b != 128 && b != 256 && b != 512))
-        return code(Status::invalid_argument);
+    if (!tsk_decode_supported(shape->group, shape->block, shape->depth)) return code(Status::invalid_argument);
     q->queue->set_shape(as<hip::LaunchShape>(*shape));
     return 0;
 }
@@ -267,9 +263,7 @@ uint32_t ts_queue_autotune(ts_queue *q, ts_mutable_bytes ws, ts_device_result *r
         std::vector<hip::LaunchShape> c(count);
         for (uint64_t i = 0; i < count; ++i) {
             c[i] = as<hip::LaunchShape>(candidates[i]);
-            const uint32_t g = c[i].group, b = c[i].block;
-            if ((g != 1 && g != 2 && g != 4 && g != 8 && g != 16) || (b != 64 && b != 128 && b != 256 && b != 512))
-                return code(Status::invalid_argument);
+            if (!tsk_decode_supported(c[i].group, c[i].block, c[i].depth)) return code(Status::invalid_argument);
         }
         std::vector<float> times;
         const Status st = q->queue->autotune(as<MutableBytes>(ws), {reinterpret_cast<DeviceResult *>(r), n},

@@ -187,12 +187,13 @@ Status stage_tiles(const CompressedInput &source, const Roi &roi, std::uint32_t 
                 staged.push_back({static_cast<std::uintptr_t>(offset), static_cast<std::uintptr_t>(offset + t.length)});
             }
             tsk_decode_task task{};
-            task.offset = offset;
-            task.tile_index = index;
+            task.payload = static_cast<const std::uint8_t *>(source.bytes.data) + offset;
+            task.column = static_cast<std::uint32_t>(col);
+            task.row = static_cast<std::uint32_t>(row);
             task.image = image;
             task.rows = TSK_NO_ROWS;
             task.words = static_cast<std::uint32_t>(t.length / 4);
-            task.mode = static_cast<std::uint32_t>(t.mode);
+            task.shape = TSK_TASK_SHAPE(static_cast<std::uint32_t>(t.mode), t.height, t.width);
             if (t.mode == TileMode::predict_bitpack) {
                 task.rows = static_cast<std::uint32_t>(d.rows.size());
                 for (std::uint64_t r = 0; r < t.height; ++r) {
@@ -259,15 +260,19 @@ Status draft_decode_item(const BatchItem &item, Operation operation, bool requir
         if (reinterpret_cast<std::uintptr_t>(source.encoder_result) % alignof(DeviceResult) != 0)
             return Status::invalid_argument;
         if (source.bytes.size < bound) return Status::buffer_too_small;
-        const std::uint64_t across = columns(image, shape);
         std::uint64_t n = 0;
         for (std::uint64_t row = roi.y / shape.height; row <= (std::uint64_t{roi.y} + roi.height - 1) / shape.height; ++row)
             for (std::uint64_t col = roi.x / shape.width; col <= (std::uint64_t{roi.x} + roi.width - 1) / shape.width;
                  ++col, ++n) {
                 tsk_decode_task task{};
-                task.tile_index = row * across + col;
+                task.payload = nullptr; // resolved on the device from the index
+                task.column = static_cast<std::uint32_t>(col);
+                task.row = static_cast<std::uint32_t>(row);
                 task.image = index;
                 task.rows = TSK_NO_ROWS;
+                const std::uint64_t tw = std::min<std::uint64_t>(shape.width, image.width - col * shape.width);
+                const std::uint64_t th = std::min<std::uint64_t>(shape.height, image.height - row * shape.height);
+                task.shape = TSK_TASK_SHAPE(0u, th, tw);
                 d.tasks.push_back(task);
             }
         d.tiles_per_image.push_back(n);
@@ -577,12 +582,14 @@ Status run_decode(const PlanState &plan, DeviceResult *results, hipStream_t stre
     a.rows = plan.rows;
     a.results = results;
     a.task_count = plan.task_count;
+    a.row_count = plan.host_rows.size();
     a.image_count = count;
     const bool calibrated = plan.operation == Operation::decode_calibrate;
     a.kind = plan.type == PixelType::uint16 ? (calibrated ? TSK_KIND_U16_CAL : TSK_KIND_U16)
                                             : (calibrated ? TSK_KIND_F32_CAL : TSK_KIND_F32);
     a.group = shape.group;
     a.block = shape.block;
+    a.depth = shape.depth;
     const std::uint64_t grid = shape.grid != 0 ? shape.grid : std::min<std::uint64_t>(plan.task_count, 1u << 20);
     a.grid = static_cast<std::uint32_t>(std::max<std::uint64_t>(grid, 1));
     if (tsk_decode(&a, stream) != hipSuccess) return hip_failure();

@@ -44,6 +44,7 @@ struct DecodeShape {
     std::uint32_t group = 8;
     std::uint32_t block = 256;
     std::uint32_t grid = 0; // 0: one workgroup per tile
+    std::uint32_t depth = 1;
 };
 
 struct PlanState {

@@ -7,7 +7,6 @@
 #include <new>
 #include <stdexcept>
 
-extern "C" int tsk_decode_occupancy(std::uint32_t kind, std::uint32_t group, std::uint32_t block, int *blocks);
 
 namespace tilestream::hip {
 
@@ -23,6 +22,7 @@ struct DecodeQueue::Impl {
     const tsk_decode_task *tasks = nullptr;
     const std::uint16_t *rows = nullptr;
     std::uint64_t task_count = 0;
+    std::uint64_t row_count = 0;
     QueueStats stats;
     LaunchShape shape;
     hipGraphExec_t graph = nullptr;
@@ -114,6 +114,7 @@ Status DecodeQueue::prepare(const BatchItem *items, const Operation *operations,
         impl->tasks = reinterpret_cast<const tsk_decode_task *>(base + tasks_at);
         impl->rows = reinterpret_cast<const std::uint16_t *>(base + rows_at);
         impl->task_count = d.tasks.size();
+        impl->row_count = d.rows.size();
         if (tsk_preload(1) != hipSuccess || tsk_preload(3) != hipSuccess) return detail::hip_failure();
         if (!upload(impl->storage, images_at, d.images, stream) || !upload(impl->storage, tasks_at, d.tasks, stream) ||
             !upload(impl->storage, rows_at, d.rows, stream) || hipStreamSynchronize(stream) != hipSuccess)
@@ -156,13 +157,14 @@ Status DecodeQueue::run(MutableBytes workspace, ResultBuffer results, hipStream_
     if (grid == 0) {
         int cus = 0, per_cu = 0;
         if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, q.device) != hipSuccess ||
-            tsk_decode_occupancy(q.kind(), sh.group, sh.block, &per_cu) != hipSuccess)
+            tsk_decode_occupancy(q.kind(), sh.group, sh.block, sh.depth, &per_cu) != hipSuccess)
             return detail::hip_failure();
         grid = static_cast<std::uint32_t>(std::max(1, cus * std::max(1, per_cu)));
     }
-    auto *counter = static_cast<std::uint32_t *>(workspace.data);
+    if (!tsk_decode_supported(sh.group, sh.block, sh.depth)) return Status::invalid_argument;
+    auto *counter = static_cast<std::uint64_t *>(workspace.data);
     if (tsk_results_begin(results.data, static_cast<std::uint32_t>(q.count), stream) != hipSuccess ||
-        (sh.chunk != 0 && tsk_fill_u32(counter, 1, 0, stream) != hipSuccess))
+        (sh.chunk != 0 && tsk_fill_u32(static_cast<std::uint32_t *>(workspace.data), 2, 0, stream) != hipSuccess))
         return detail::hip_failure();
     tsk_decode_args a{};
     a.images = q.images;
@@ -171,12 +173,14 @@ Status DecodeQueue::run(MutableBytes workspace, ResultBuffer results, hipStream_
     a.results = results.data;
     a.queue = sh.chunk != 0 ? counter : nullptr;
     a.task_count = q.task_count;
+    a.row_count = q.row_count;
     a.image_count = static_cast<std::uint32_t>(q.count);
     a.kind = q.kind();
     a.group = sh.group;
     a.block = sh.block;
     a.grid = grid;
     a.chunk = sh.chunk;
+    a.depth = sh.depth;
     if (tsk_decode(&a, stream) != hipSuccess ||
         tsk_results_complete(q.images, static_cast<std::uint32_t>(q.count), results.data, stream) != hipSuccess)
         return detail::hip_failure();
