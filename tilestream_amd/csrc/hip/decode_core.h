@@ -40,6 +40,38 @@ constexpr std::uint32_t kPayloadWords = 4096;
 constexpr std::uint32_t kMaxBlocks = (kPayloadWords + 3 + 3) / 4;
 constexpr std::uint32_t kStageWords = kMaxBlocks * 4 + 280;
 
+// The descriptor tables never change while a decode runs.  Reading them through
+// the constant address space lets the compiler use scalar loads (the scalar
+// cache) instead of vector loads whose waits would also wait for the payload
+// prefetch in flight.
+// Global-memory pointers.  Pointers read from descriptor tables are generic to
+// the compiler, and generic (FLAT) accesses also count against the LDS wait
+// counter, so every LDS wait would drain the payload prefetch and the output
+// stores.  Casting them to the global address space keeps the two apart.
+template <typename T> using gptr = T __attribute__((address_space(1))) *;
+
+// Native vector types: unlike HIP's uint4/float4 classes they load and store
+// through address-space-qualified pointers.
+typedef std::uint32_t u32x4 __attribute__((ext_vector_type(4)));
+typedef std::uint32_t u32x2 __attribute__((ext_vector_type(2)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+template <typename T> __device__ inline gptr<T> global(T *p) {
+    return reinterpret_cast<gptr<T>>(reinterpret_cast<std::uintptr_t>(p));
+}
+
+template <typename T> __device__ inline T load_constant(const T *table, std::uint64_t index) {
+    static_assert(sizeof(T) % 4 == 0, "descriptor records are whole words");
+    using Word = const __attribute__((address_space(4))) std::uint32_t;
+    Word *src = reinterpret_cast<Word *>(reinterpret_cast<std::uintptr_t>(table + index));
+    std::uint32_t words[sizeof(T) / 4];
+#pragma unroll
+    for (unsigned k = 0; k < sizeof(T) / 4; ++k) words[k] = src[k];
+    T out;
+    __builtin_memcpy(&out, words, sizeof(T));
+    return out;
+}
+
 struct ResultBits {
     std::uint32_t status;
     std::uint32_t reserved;
@@ -101,7 +133,7 @@ template <int G> struct Group {
 };
 
 struct Region {
-    std::uint8_t *line;   // output row of this tile row
+    gptr<std::uint8_t> line; // output row of this tile row
     std::int64_t ex_tile; // output column of tile column 0 (may be negative)
     std::uint64_t ey;     // output row index
     std::uint32_t left, right;
@@ -118,31 +150,31 @@ __device__ inline void emit(const tsk_decode_image &img, const Region &r, std::u
     const std::int64_t ex0 = r.ex_tile + col0;
     if constexpr (CAL) {
         float out[G];
-        const std::uint8_t *orow = img.offset_map ? img.offset_map + r.ey * img.offset_pitch : nullptr;
-        const std::uint8_t *grow = img.gain_map ? img.gain_map + r.ey * img.gain_pitch : nullptr;
+        const gptr<const std::uint8_t> orow = img.offset_map ? global(img.offset_map) + r.ey * img.offset_pitch : nullptr;
+        const gptr<const std::uint8_t> grow = img.gain_map ? global(img.gain_map) + r.ey * img.gain_pitch : nullptr;
 #pragma unroll
         for (int k = 0; k < G; ++k) {
             const std::uint32_t c = col0 + k;
             float o = img.offset, gn = img.gain;
             if (c >= r.left && c < r.right) {
-                if (orow) o = reinterpret_cast<const float *>(orow)[ex0 + k];
-                if (grow) gn = reinterpret_cast<const float *>(grow)[ex0 + k];
+                if (orow) o = reinterpret_cast<gptr<const float>>(orow)[ex0 + k];
+                if (grow) gn = reinterpret_cast<gptr<const float>>(grow)[ex0 + k];
             }
             // Two roundings to nearest even, never contracted into an FMA.
             out[k] = __fmul_rn(__fsub_rn(sample_float(g.v[k], U16), o), gn);
         }
-        float *dst = reinterpret_cast<float *>(r.line) + ex0;
+        gptr<float> dst = reinterpret_cast<gptr<float>>(r.line) + ex0;
         if (inside && G >= 4 && (reinterpret_cast<std::uintptr_t>(dst) & 15u) == 0) {
 #pragma unroll
             for (int k = 0; k + 4 <= G; k += 4)
-                *reinterpret_cast<float4 *>(dst + k) = make_float4(out[k], out[k + 1], out[k + 2], out[k + 3]);
+                *reinterpret_cast<gptr<f32x4>>(dst + k) = f32x4{out[k], out[k + 1], out[k + 2], out[k + 3]};
         } else {
 #pragma unroll
             for (int k = 0; k < G; ++k)
                 if (col0 + k >= r.left && col0 + k < r.right) dst[k] = out[k];
         }
     } else if constexpr (U16) {
-        std::uint16_t *dst = reinterpret_cast<std::uint16_t *>(r.line) + ex0;
+        gptr<std::uint16_t> dst = reinterpret_cast<gptr<std::uint16_t>>(r.line) + ex0;
         constexpr int kBytes = 2 * G;
         if (G >= 2 && inside && (reinterpret_cast<std::uintptr_t>(dst) & ((kBytes >= 16 ? 16 : kBytes) - 1)) == 0) {
             std::uint32_t w[G / 2 > 0 ? G / 2 : 1];
@@ -151,11 +183,11 @@ __device__ inline void emit(const tsk_decode_image &img, const Region &r, std::u
             if constexpr (G >= 8) {
 #pragma unroll
                 for (int i = 0; i + 4 <= G / 2; i += 4)
-                    *reinterpret_cast<uint4 *>(dst + 2 * i) = make_uint4(w[i], w[i + 1], w[i + 2], w[i + 3]);
+                    *reinterpret_cast<gptr<u32x4>>(dst + 2 * i) = u32x4{w[i], w[i + 1], w[i + 2], w[i + 3]};
             } else if constexpr (G == 4) {
-                *reinterpret_cast<uint2 *>(dst) = make_uint2(w[0], w[1]);
+                *reinterpret_cast<gptr<u32x2>>(dst) = u32x2{w[0], w[1]};
             } else if constexpr (G == 2) {
-                *reinterpret_cast<std::uint32_t *>(dst) = w[0];
+                *reinterpret_cast<gptr<std::uint32_t>>(dst) = w[0];
             }
         } else {
 #pragma unroll
@@ -163,15 +195,15 @@ __device__ inline void emit(const tsk_decode_image &img, const Region &r, std::u
                 if (col0 + k >= r.left && col0 + k < r.right) dst[k] = static_cast<std::uint16_t>(g.v[k]);
         }
     } else {
-        std::uint32_t *dst = reinterpret_cast<std::uint32_t *>(r.line) + ex0;
+        gptr<std::uint32_t> dst = reinterpret_cast<gptr<std::uint32_t>>(r.line) + ex0;
         constexpr int kBytes = 4 * G;
         if (inside && (reinterpret_cast<std::uintptr_t>(dst) & ((kBytes >= 16 ? 16 : kBytes) - 1)) == 0) {
             if constexpr (G >= 4) {
 #pragma unroll
                 for (int k = 0; k + 4 <= G; k += 4)
-                    *reinterpret_cast<uint4 *>(dst + k) = make_uint4(g.v[k], g.v[k + 1], g.v[k + 2], g.v[k + 3]);
+                    *reinterpret_cast<gptr<u32x4>>(dst + k) = u32x4{g.v[k], g.v[k + 1], g.v[k + 2], g.v[k + 3]};
             } else if constexpr (G == 2) {
-                *reinterpret_cast<uint2 *>(dst) = make_uint2(g.v[0], g.v[1]);
+                *reinterpret_cast<gptr<u32x2>>(dst) = u32x2{g.v[0], g.v[1]};
             } else {
                 dst[0] = g.v[0];
             }
@@ -204,7 +236,7 @@ __device__ void decode_tile(const tsk_decode_image &img, const TileView &t, cons
     for (std::uint32_t row = t.top + threadIdx.x / S; row < t.bottom; row += rows_per_step) {
         Region r;
         r.ey = t.ty + row - img.roi_y;
-        r.line = img.out + r.ey * img.out_pitch;
+        r.line = global(img.out) + r.ey * img.out_pitch;
         r.ex_tile = static_cast<std::int64_t>(t.tx) - static_cast<std::int64_t>(img.roi_x);
         r.left = t.left;
         r.right = t.right;
@@ -273,13 +305,13 @@ struct Pending {
     std::uint32_t state; // 0 decode, 1 producer failed (skip), 2 malformed
     std::uint32_t mis;   // words ahead of the payload in its first 16-byte block
     std::uint32_t nblocks;
-    const uint4 *blocks;
+    gptr<const u32x4> blocks;
 };
 
 // Reads a task descriptor and, on the trusted path, the producer's result and
 // the device-resident index entry (reference decoder.cu:191-220).
 __device__ inline Pending describe(const tsk_decode_args &a, std::uint64_t i) {
-    const tsk_decode_task task = a.tasks[i];
+    const tsk_decode_task task = load_constant(a.tasks, i);
     Pending p;
     p.image = task.image;
     p.column = task.column;
@@ -292,13 +324,14 @@ __device__ inline Pending describe(const tsk_decode_args &a, std::uint64_t i) {
     p.state = 0;
     const std::uint8_t *addr = task.payload;
     if (addr == nullptr) {
-        const tsk_decode_image &img = a.images[task.image];
-        const auto *prod = static_cast<const ResultBits *>(img.producer);
+        const tsk_decode_image img = load_constant(a.images, task.image);
+        const gptr<const ResultBits> prod = global(static_cast<const ResultBits *>(img.producer));
         if (prod->status != 0) {
             p.state = 1;
         } else {
             const std::uint64_t index = std::uint64_t{task.row} * img.tile_columns + task.column;
-            const auto *s32 = reinterpret_cast<const std::uint32_t *>(img.src + 64 + index * 32);
+            const gptr<const std::uint32_t> s32 =
+                global(reinterpret_cast<const std::uint32_t *>(img.src + 64 + index * 32));
             const std::uint32_t mode = s32[0] & 0xffu;
             const std::uint32_t dims = s32[1];
             const std::uint64_t off = s32[2] | (std::uint64_t{s32[3]} << 32);
@@ -321,7 +354,7 @@ __device__ inline Pending describe(const tsk_decode_args &a, std::uint64_t i) {
     }
     const std::uintptr_t u = reinterpret_cast<std::uintptr_t>(addr);
     p.mis = static_cast<std::uint32_t>((u >> 2) & 3u);
-    p.blocks = reinterpret_cast<const uint4 *>(u - 4u * p.mis);
+    p.blocks = reinterpret_cast<gptr<const u32x4>>(u - 4u * p.mis);
     p.nblocks = p.state == 0 ? (p.words + p.mis + 3) >> 2 : 0;
     return p;
 }
@@ -347,7 +380,11 @@ __device__ inline std::uint64_t draw(const tsk_decode_args &a, Cursor &c, std::u
             c.ahead = atomicAdd(reinterpret_cast<unsigned long long *>(a.queue), a.chunk);
         }
         __syncthreads();
-        c.next = shared_next[0];
+        // The chunk start is the same for every lane: keep it (and every
+        // descriptor read through it) in scalar registers.
+        const std::uint64_t start = shared_next[0];
+        c.next = (static_cast<std::uint64_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(start >> 32))) << 32) |
+                 static_cast<std::uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(start)));
         c.end = c.next + a.chunk;
     }
     return c.next++;
@@ -371,13 +408,14 @@ __global__ void __launch_bounds__(BLOCK) decode_kernel(tsk_decode_args a) {
     if (i >= a.task_count) return;
 
     Pending p = describe(a, i);
-    uint4 reg[KMAX];
+    u32x4 reg[KMAX];
     std::uint32_t row_reg = 0;
     if constexpr (DEPTH > 1) {
 #pragma unroll
         for (int k = 0; k < KMAX; ++k)
             if (tid + k * BLOCK < p.nblocks) reg[k] = p.blocks[tid + k * BLOCK];
-        if (p.mode == 2 && p.rows != TSK_NO_ROWS && tid < p.h && p.rows + tid < a.row_count) row_reg = a.rows[p.rows + tid];
+        if (p.mode == 2 && p.rows != TSK_NO_ROWS && tid < p.h && p.rows + tid < a.row_count)
+            row_reg = (global(a.rows) + p.rows)[tid];
     }
 
     for (;;) {
@@ -386,13 +424,14 @@ __global__ void __launch_bounds__(BLOCK) decode_kernel(tsk_decode_args a) {
         if constexpr (DEPTH > 1) {
 #pragma unroll
             for (int k = 0; k < KMAX; ++k)
-                if (tid + k * BLOCK < c.nblocks) *reinterpret_cast<uint4 *>(stage + 4 * (tid + k * BLOCK)) = reg[k];
-            if (tid < 64) row_start[tid] = static_cast<std::uint16_t>(row_reg);
+                if (tid + k * BLOCK < c.nblocks) *reinterpret_cast<u32x4 *>(stage + 4 * (tid + k * BLOCK)) = reg[k];
+            if (c.mode == 2 && c.rows != TSK_NO_ROWS && tid < c.h && c.rows + tid < a.row_count)
+                row_start[tid] = static_cast<std::uint16_t>(row_reg);
         } else {
             for (std::uint32_t k = tid; k < c.nblocks; k += BLOCK)
-                *reinterpret_cast<uint4 *>(stage + 4 * k) = c.blocks[k];
+                *reinterpret_cast<u32x4 *>(stage + 4 * k) = c.blocks[k];
             if (c.mode == 2 && c.rows != TSK_NO_ROWS && tid < c.h && c.rows + tid < a.row_count)
-                row_start[tid] = a.rows[c.rows + tid];
+                row_start[tid] = (global(a.rows) + c.rows)[tid];
         }
         __syncthreads();
 
@@ -405,9 +444,8 @@ __global__ void __launch_bounds__(BLOCK) decode_kernel(tsk_decode_args a) {
 #pragma unroll
                 for (int k = 0; k < KMAX; ++k)
                     if (tid + k * BLOCK < p.nblocks) reg[k] = p.blocks[tid + k * BLOCK];
-                row_reg = 0;
                 if (p.mode == 2 && p.rows != TSK_NO_ROWS && tid < p.h && p.rows + tid < a.row_count)
-                    row_reg = a.rows[p.rows + tid];
+                    row_reg = (global(a.rows) + p.rows)[tid];
             }
         }
 
@@ -415,7 +453,7 @@ __global__ void __launch_bounds__(BLOCK) decode_kernel(tsk_decode_args a) {
             if (tid == 0) fail(a.results, c.image, kMalformed);
         } else if (c.state == 0) {
             // The descriptor is copied by value: output stores cannot alias it.
-            const tsk_decode_image img = a.images[c.image];
+            const tsk_decode_image img = load_constant(a.images, c.image);
             const bool u16 = img.type == TSK_UINT16;
             bool good = true;
             if (c.mode == 2 && c.rows == TSK_NO_ROWS) {
