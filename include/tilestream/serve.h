@@ -1,0 +1,113 @@
+// Serving resident archives: an extension of the reference API (no counterpart
+// in the reference, which plans every request on the host with prepare_batch).
+//
+// ArchiveSet keeps archives resident in device memory together with their tile
+// and row tables, planned once when an archive is added.  A batch of region
+// requests then uploads only small per-request records; the decode kernel
+// derives each request's tiles on the device.  Server runs continuous
+// batching over several streams: requests that arrive while batches are on
+// the GPU are admitted as stream slots and output memory free up, and
+// batches on different streams overlap on the device.
+#pragma once
+
+#include <tilestream/batch.h>
+#include <tilestream/hip.h>
+
+#include <cstdint>
+#include <memory>
+
+namespace tilestream::hip {
+
+struct ArchiveInfo {
+    ImageSpec image;
+    TileShape tile;
+    std::uint64_t tile_count = 0;
+    std::uint64_t file_bytes = 0;
+};
+
+class ArchiveSet {
+  public:
+    ~ArchiveSet();
+    ArchiveSet(const ArchiveSet &) = delete;
+    ArchiveSet &operator=(const ArchiveSet &) = delete;
+
+    [[nodiscard]] static Status create(std::unique_ptr<ArchiveSet> &set) noexcept;
+
+    // Adds a complete archive already in device memory of the current device,
+    // validated on the host into `metadata` (parse_metadata of the same bytes).
+    // The bytes must stay allocated and unchanged while the set is in use.
+    [[nodiscard]] Status add(ConstBytes device_file, const ImageMetadata &metadata, std::uint32_t &id) noexcept;
+    // Uploads the tables of every archive added so far (synchronizes `stream`).
+    [[nodiscard]] Status commit(hipStream_t stream) noexcept;
+
+    [[nodiscard]] std::uint32_t size() const noexcept;
+    [[nodiscard]] Status info(std::uint32_t id, ArchiveInfo &out) const noexcept;
+
+    struct Impl;
+    [[nodiscard]] const Impl *impl() const noexcept { return impl_.get(); }
+
+  private:
+    explicit ArchiveSet(std::unique_ptr<Impl> impl) noexcept;
+    std::unique_ptr<Impl> impl_;
+};
+
+// A region of a resident archive, decoded bit-exact (calibrated = false) or
+// fused with scalar calibration into FP32.
+struct RegionRequest {
+    std::uint32_t archive = 0;
+    std::uint32_t calibrated = 0;
+    Roi roi;
+    ImageView output; // device; archive pixel type, or float32 when calibrated
+    float offset = 0.0F;
+    float gain = 1.0F;
+};
+
+struct ServerOptions {
+    std::uint32_t streams = 3;           // batches in flight at once
+    std::uint32_t max_batch = 1024;      // requests per batch
+    std::uint32_t max_tasks = 1u << 22;  // tiles per batch (a larger request is rejected)
+    std::uint64_t admission_bytes = 200ull << 30; // output bytes in flight
+    LaunchShape shape;                   // decode kernel launch shape (grid 0: sized per batch)
+};
+
+struct ServerStats {
+    std::uint64_t requests = 0;
+    std::uint64_t batches = 0;
+    std::uint64_t decoded_bytes = 0;     // region pixels x stored sample bytes
+    std::uint64_t failed = 0;            // requests whose device result was not success
+    double seconds = 0.0;                // first arrival to last completion
+};
+
+class Server {
+  public:
+    ~Server();
+    Server(const Server &) = delete;
+    Server &operator=(const Server &) = delete;
+
+    [[nodiscard]] static Status create(const ArchiveSet &archives, const ServerOptions &options,
+                                       std::unique_ptr<Server> &server) noexcept;
+
+    // Validates a request against the archive set (no device work).
+    [[nodiscard]] Status check(const RegionRequest &request) const noexcept;
+
+    // Open-loop replay: request i arrives arrival_ns[i] nanoseconds after the
+    // start (non-decreasing).  latency_ns[i] (optional) receives completion -
+    // arrival as seen by the host, statuses[i] (optional) the request's device
+    // status.  Every request is validated before the replay starts; outputs must
+    // be pairwise disjoint and disjoint from the archives.
+    [[nodiscard]] Status replay(const RegionRequest *requests, const std::uint64_t *arrival_ns,
+                                std::uint64_t count, std::uint64_t *latency_ns, std::uint32_t *statuses,
+                                ServerStats &stats) noexcept;
+
+    // Decodes `count` requests, all arriving at once, and waits for them.
+    [[nodiscard]] Status run_batch(const RegionRequest *requests, std::uint64_t count,
+                                   std::uint32_t *statuses) noexcept;
+
+    struct Impl;
+
+  private:
+    explicit Server(std::unique_ptr<Impl> impl) noexcept;
+    std::unique_ptr<Impl> impl_;
+};
+
+} // namespace tilestream::hip

@@ -146,6 +146,44 @@ uint32_t ts_queue_autotune(ts_queue *queue, ts_mutable_bytes workspace, ts_devic
                            uint64_t result_count, void *stream, const ts_launch_shape *candidates,
                            uint64_t candidate_count, uint32_t repeats, float *milliseconds);
 
+/* ---- extension: serving resident archives (tilestream/serve.h) ---------
+ * No counterpart in the reference.  Archives stay in device memory with tile
+ * tables planned once; region requests are batched continuously over several
+ * streams. */
+typedef struct ts_archive_set ts_archive_set;
+typedef struct ts_server ts_server;
+typedef struct { ts_image_spec image; ts_tile_shape tile; uint64_t tile_count, file_bytes; } ts_archive_info;
+typedef struct {
+    uint32_t archive, calibrated;  /* calibrated: FP32 (x - offset) * gain output */
+    ts_roi roi;
+    ts_image_view output;
+    float offset, gain;
+} ts_region_request;
+typedef struct {
+    uint32_t streams, max_batch, max_tasks, reserved;
+    uint64_t admission_bytes;
+    ts_launch_shape shape;
+} ts_server_options;
+typedef struct { uint64_t requests, batches, decoded_bytes, failed; double seconds; } ts_server_stats;
+uint32_t ts_archive_set_create(ts_archive_set **set);
+void ts_archive_set_destroy(ts_archive_set *set);
+/* `metadata` is the parse of the same bytes; the bytes stay resident and unchanged. */
+uint32_t ts_archive_set_add(ts_archive_set *set, ts_const_bytes device_file, const ts_metadata *metadata,
+                            uint32_t *id);
+uint32_t ts_archive_set_commit(ts_archive_set *set, void *stream);
+uint32_t ts_archive_set_size(const ts_archive_set *set);
+uint32_t ts_archive_set_info(const ts_archive_set *set, uint32_t id, ts_archive_info *out);
+void ts_server_default_options(ts_server_options *out);
+/* The archive set must outlive the server. */
+uint32_t ts_server_create(const ts_archive_set *set, const ts_server_options *options, ts_server **server);
+void ts_server_destroy(ts_server *server);
+uint32_t ts_server_check(const ts_server *server, const ts_region_request *request);
+/* Open-loop replay; arrival_ns non-decreasing; latency_ns and statuses may be NULL. */
+uint32_t ts_server_replay(ts_server *server, const ts_region_request *requests, const uint64_t *arrival_ns,
+                          uint64_t count, uint64_t *latency_ns, uint32_t *statuses, ts_server_stats *stats);
+uint32_t ts_server_run_batch(ts_server *server, const ts_region_request *requests, uint64_t count,
+                             uint32_t *statuses);
+
 /* ---- HIP runtime helpers for hosts without their own HIP binding --------
  * These return hipError_t values (0 = hipSuccess). */
 int ts_hip_malloc(uint64_t bytes, void **data);

@@ -73,6 +73,22 @@ typedef struct {
 
 #define TSK_TASK_SHAPE(mode, height, width) ((uint32_t)(mode) | ((uint32_t)(height) << 8) | ((uint32_t)(width) << 16))
 
+/* Request mode: a batch of region requests over resident archives.  Request r
+ * covers tasks [first_task, next request's first_task): the tiles of columns
+ * col0.. col0+cols-1 and rows row0.., row-major, of the archive whose tiles
+ * start at tile_base in the resident tile table (tsk_decode_args.tasks). */
+typedef struct {
+    uint64_t first_task;
+    uint32_t tile_base;          /* index of the archive's tile 0 in the tile table */
+    uint32_t tile_columns;       /* tile columns of the archive */
+    uint32_t col0, row0;         /* first tile of the region */
+    uint32_t cols;               /* tiles per region row */
+    uint32_t reserved;
+} tsk_request;
+
+/* Every `TSK_REQUEST_STRIDE` tasks, the request holding the first of them. */
+#define TSK_REQUEST_STRIDE 64u
+
 typedef struct {
     const tsk_decode_image *images;
     const tsk_decode_task *tasks;
@@ -88,7 +104,9 @@ typedef struct {
     uint32_t grid;               /* workgroups; tiles are strided or drawn from the queue */
     uint32_t chunk;              /* tiles taken per queue draw */
     uint32_t depth;              /* tiles a workgroup holds at once: 1 staged, 2 = + next prefetched */
-    uint32_t reserved;
+    uint32_t request_count;      /* request mode when non-zero (requests, request_index set) */
+    const tsk_request *requests;
+    const uint32_t *request_index; /* task / TSK_REQUEST_STRIDE -> request */
 } tsk_decode_args;
 
 /* Per-image descriptor of an encode job. */

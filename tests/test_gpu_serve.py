@@ -1,0 +1,224 @@
+"""Serving resident archives: region requests decoded in request mode (tiles
+derived on the device from each request's region), continuous batching over
+several streams, each result identical alone and in the mix, request checks."""
+import numpy as np
+import pytest
+
+import tilestream_amd as ts
+from tilestream_amd import api, generators as g
+
+pytestmark = pytest.mark.gpu
+
+ARCHIVES = [("u16", 700, 389, (64, 64)), ("f32", 517, 301, (128, 32)), ("u16", 1000, 600, (256, 16)),
+            ("f32", 300, 200, (64, 64)), ("u16", 129, 33, (128, 32))]
+OFFSET, GAIN = 100.0, 1.05
+
+
+class Resident:
+    def __init__(self, orc):
+        self.pixels, self.files, self.mems = [], [], []
+        self.set = api.ArchiveSet()
+        for i, (kind, w, h, tile) in enumerate(ARCHIVES):
+            px = g.detector_u16(w, h, 300 + i) if kind == "u16" else g.field_f32(w, h, 300 + i)
+            data = orc.compress(px, tile)
+            st, meta = api.parse_metadata(data)
+            assert st == ts.Status.success
+            mem = api.DeviceMemory.from_host(np.frombuffer(data, np.uint8))
+            st, ident = self.set.add(mem, meta)
+            assert st == ts.Status.success and ident == i
+            self.pixels.append(px)
+            self.files.append(data)
+            self.mems.append(mem)
+        assert self.set.commit() == ts.Status.success
+
+
+@pytest.fixture(scope="module")
+def resident(orc):
+    return Resident(orc)
+
+
+def _requests(resident, rng, count, big_every=7):
+    """Random regions: mostly small (one or a few tiles), some large."""
+    specs = []
+    for k in range(count):
+        a = int(rng.integers(0, len(ARCHIVES)))
+        h, w = resident.pixels[a].shape
+        if k % big_every == 0:
+            rw, rh = int(rng.integers(w // 2, w + 1)), int(rng.integers(h // 2, h + 1))
+        else:
+            rw, rh = int(rng.integers(1, min(w, 90) + 1)), int(rng.integers(1, min(h, 70) + 1))
+        x, y = int(rng.integers(0, w - rw + 1)), int(rng.integers(0, h - rh + 1))
+        specs.append((a, (x, y, rw, rh), bool(rng.integers(0, 2))))
+    return specs
+
+
+class Outputs:
+    """Disjoint device outputs for a list of request specs, in one allocation."""
+
+    def __init__(self, resident, specs):
+        self.offsets, total = [], 0
+        for a, (x, y, w, h), cal in specs:
+            s = 4 if cal or resident.pixels[a].dtype != np.uint16 else 2
+            self.offsets.append((total, s))
+            total += (w * h * s + 255) // 256 * 256
+        self.mem = api.DeviceMemory(max(256, total))
+        self.mem.fill(0xA5)
+        self.requests = []
+        for (a, roi, cal), (off, s) in zip(specs, self.offsets):
+            ptype = ts.PixelType.float32 if s == 4 else ts.PixelType.uint16
+            view = api.device_view(self.mem, ptype, roi[2], roi[3], offset=off, capacity=roi[2] * roi[3] * s)
+            self.requests.append(api.region_request(a, roi, view, cal, OFFSET, GAIN))
+
+    def read(self, specs, i):
+        off, s = self.offsets[i]
+        w, h = specs[i][1][2], specs[i][1][3]
+        raw = self.mem.download(w * h * s, off)
+        return raw.view(np.uint16 if s == 2 else np.uint32).reshape(h, w)
+
+
+def _expected(orc, resident, spec):
+    a, roi, cal = spec
+    if cal:
+        return orc.decode_roi_calibrate(resident.files[a], roi, OFFSET, GAIN).view(np.uint32)
+    x, y, w, h = roi
+    px = resident.pixels[a][y:y + h, x:x + w]
+    return px if px.dtype == np.uint16 else px.view(np.uint32)
+
+
+def _same(got, want):
+    if got.dtype == np.uint32:
+        fg, fw = got.view(np.float32), want.view(np.float32)
+        return bool(np.all((got == want) | (np.isnan(fg) & np.isnan(fw))))
+    return bool(np.array_equal(got, want))
+
+
+@pytest.mark.parametrize("shape", [(8, 256, 0, 4, 2), (4, 128, 0, 0, 1), (16, 512, 0, 2, 2), (8, 256, 3, 1, 1)])
+def test_batch_matches_oracle(orc, resident, shape):
+    rng = np.random.default_rng(500 + shape[0] + shape[1])
+    specs = _requests(resident, rng, 90)
+    out = Outputs(resident, specs)
+    server = api.Server(resident.set, api.server_options(streams=3, max_batch=16, shape=shape))
+    st, statuses = server.run_batch(out.requests)
+    assert st == ts.Status.success
+    assert (statuses == 0).all()
+    for i, spec in enumerate(specs):
+        assert _same(out.read(specs, i), _expected(orc, resident, spec)), (shape, i, spec)
+
+
+def test_many_single_tile_requests_in_one_batch(orc, resident):
+    """Hundreds of one-tile requests share each sampled request-index stride."""
+    rng = np.random.default_rng(77)
+    specs = []
+    for k in range(700):
+        a = k % len(ARCHIVES)
+        h, w = resident.pixels[a].shape
+        tw, th = ARCHIVES[a][3]
+        cx, cy = int(rng.integers(0, (w + tw - 1) // tw)), int(rng.integers(0, (h + th - 1) // th))
+        x0, y0 = cx * tw, cy * th
+        rw, rh = min(tw, w - x0), min(th, h - y0)
+        specs.append((a, (x0, y0, rw, rh), k % 3 == 0))
+    specs.append((2, (0, 0, 1000, 600), False))  # 4 x 38 = 152 tiles after them
+    out = Outputs(resident, specs)
+    server = api.Server(resident.set, api.server_options(streams=2, max_batch=1024))
+    st, statuses = server.run_batch(out.requests)
+    assert st == ts.Status.success and (statuses == 0).all()
+    for i, spec in enumerate(specs):
+        assert _same(out.read(specs, i), _expected(orc, resident, spec)), (i, spec)
+
+
+def test_alone_and_in_the_mix_are_identical(resident):
+    rng = np.random.default_rng(91)
+    specs = _requests(resident, rng, 60)
+    mix = Outputs(resident, specs)
+    server = api.Server(resident.set, api.server_options(streams=3, max_batch=8))
+    st, statuses = server.run_batch(mix.requests)
+    assert st == ts.Status.success and (statuses == 0).all()
+    for i in range(0, len(specs), 4):
+        alone = Outputs(resident, [specs[i]])
+        st, statuses = server.run_batch(alone.requests)
+        assert st == ts.Status.success and statuses[0] == 0
+        assert np.array_equal(alone.read([specs[i]], 0), mix.read(specs, i)), (i, specs[i])
+
+
+def test_replay_with_poisson_arrivals(orc, resident):
+    rng = np.random.default_rng(12)
+    specs = _requests(resident, rng, 200)
+    out = Outputs(resident, specs)
+    arrivals = np.cumsum(rng.exponential(20_000.0, len(specs))).astype(np.uint64)  # 50k requests/s
+    server = api.Server(resident.set, api.server_options(streams=3, max_batch=32,
+                                                         admission_bytes=4 << 20))
+    st, latency, statuses, stats = server.replay(out.requests, arrivals)
+    assert st == ts.Status.success
+    assert (statuses == 0).all() and stats.failed == 0
+    assert stats.requests == len(specs) and 1 <= stats.batches <= len(specs)
+    assert (latency > 0).all() and stats.seconds > 0
+    want_bytes = sum(r[1][2] * r[1][3] * resident.pixels[r[0]].itemsize for r in specs)
+    assert stats.decoded_bytes == want_bytes
+    for i, spec in enumerate(specs):
+        assert _same(out.read(specs, i), _expected(orc, resident, spec)), (i, spec)
+
+
+def test_admission_smaller_than_one_request_still_progresses(orc, resident):
+    rng = np.random.default_rng(13)
+    specs = _requests(resident, rng, 20, big_every=2)
+    out = Outputs(resident, specs)
+    server = api.Server(resident.set, api.server_options(streams=2, admission_bytes=1024))
+    st, latency, statuses, stats = server.replay(out.requests, np.zeros(len(specs), np.uint64))
+    assert st == ts.Status.success and (statuses == 0).all()
+    assert stats.batches >= len(specs) // 2  # the large regions go one at a time
+    for i, spec in enumerate(specs):
+        assert _same(out.read(specs, i), _expected(orc, resident, spec))
+
+
+def test_request_checks(resident):
+    server = api.Server(resident.set, api.server_options(max_tasks=64))
+    mem = api.DeviceMemory(2 << 20)
+    ok = api.region_request(0, (0, 0, 64, 64), api.device_view(mem, ts.PixelType.uint16, 64, 64))
+    assert server.check(ok) == ts.Status.success
+    bad_id = api.region_request(len(ARCHIVES), (0, 0, 64, 64), api.device_view(mem, ts.PixelType.uint16, 64, 64))
+    assert server.check(bad_id) == ts.Status.invalid_argument
+    out_of_image = api.region_request(0, (650, 0, 64, 64), api.device_view(mem, ts.PixelType.uint16, 64, 64))
+    assert server.check(out_of_image) == ts.Status.invalid_roi
+    empty = api.region_request(0, (0, 0, 0, 64), api.device_view(mem, ts.PixelType.uint16, 1, 64))
+    assert server.check(empty) == ts.Status.invalid_roi
+    wrong_type = api.region_request(0, (0, 0, 64, 64), api.device_view(mem, ts.PixelType.float32, 64, 64))
+    assert server.check(wrong_type) == ts.Status.invalid_argument
+    cal_u16 = api.region_request(0, (0, 0, 64, 64), api.device_view(mem, ts.PixelType.uint16, 64, 64), True)
+    assert server.check(cal_u16) == ts.Status.invalid_argument
+    wrong_size = api.region_request(0, (0, 0, 64, 64), api.device_view(mem, ts.PixelType.uint16, 63, 64))
+    assert server.check(wrong_size) == ts.Status.invalid_argument
+    too_many_tiles = api.region_request(2, (0, 0, 1000, 600), api.device_view(mem, ts.PixelType.uint16, 1000, 600))
+    assert server.check(too_many_tiles) == ts.Status.resource_limit
+    # outputs of one replay must not overlap
+    a = api.region_request(0, (0, 0, 64, 64), api.device_view(mem, ts.PixelType.uint16, 64, 64))
+    b = api.region_request(0, (64, 0, 64, 64), api.device_view(mem, ts.PixelType.uint16, 64, 64, offset=4096))
+    st, _ = server.run_batch([a, b])
+    assert st == ts.Status.invalid_argument
+    # nor write into an archive
+    arch = resident.mems[0]
+    c = api.region_request(0, (0, 0, 8, 8), api.device_view(arch, ts.PixelType.uint16, 8, 8, offset=128))
+    st, _ = server.run_batch([c])
+    assert st == ts.Status.invalid_argument
+    # arrivals must not decrease
+    d = api.region_request(0, (64, 0, 64, 64), api.device_view(mem, ts.PixelType.uint16, 64, 64, offset=16384))
+    st, *_ = server.replay([ok, d], np.array([5, 1], np.uint64))
+    assert st == ts.Status.invalid_argument
+
+
+def test_archive_set_rejects_bad_archives(orc):
+    px = g.detector_u16(200, 100, 3)
+    data = bytearray(orc.compress(px, (64, 64)))
+    st, meta = api.parse_metadata(bytes(data))
+    assert st == ts.Status.success
+    s = api.ArchiveSet()
+    mem = api.DeviceMemory.from_host(np.frombuffer(bytes(data), np.uint8))
+    st, _ = s.add(mem, meta, size=len(data) - 4)
+    assert st == ts.Status.buffer_too_small
+    t = meta.tile(1)
+    t.length += 4
+    meta.set_tile(1, t)
+    st, _ = s.add(mem, meta)
+    assert st == ts.Status.malformed_data
+    assert len(s) == 0
+    with pytest.raises(RuntimeError):
+        api.Server(s)  # nothing committed

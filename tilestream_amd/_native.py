@@ -129,6 +129,25 @@ class QueueStats(C.Structure):
                 ("output_bytes", C.c_uint64), ("payload_bytes", C.c_uint64), ("map_bytes", C.c_uint64)]
 
 
+class ArchiveInfo(C.Structure):
+    _fields_ = [("image", ImageSpec), ("tile", TileShape), ("tile_count", C.c_uint64), ("file_bytes", C.c_uint64)]
+
+
+class RegionRequest(C.Structure):
+    _fields_ = [("archive", C.c_uint32), ("calibrated", C.c_uint32), ("roi", Roi), ("output", ImageView),
+                ("offset", C.c_float), ("gain", C.c_float)]
+
+
+class ServerOptions(C.Structure):
+    _fields_ = [("streams", C.c_uint32), ("max_batch", C.c_uint32), ("max_tasks", C.c_uint32),
+                ("reserved", C.c_uint32), ("admission_bytes", C.c_uint64), ("shape", LaunchShape)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("requests", C.c_uint64), ("batches", C.c_uint64), ("decoded_bytes", C.c_uint64),
+                ("failed", C.c_uint64), ("seconds", C.c_double)]
+
+
 RECORDS = {
     "ts_image_spec": ImageSpec, "ts_tile_shape": TileShape, "ts_roi": Roi,
     "ts_const_bytes": ConstBytes, "ts_mutable_bytes": MutableBytes,
@@ -139,6 +158,8 @@ RECORDS = {
     "ts_parameter": Parameter, "ts_calibration": Calibration, "ts_tile_mapping": TileMapping,
     "ts_compressed_input": CompressedInput, "ts_batch_item": BatchItem,
     "ts_batch_request": BatchRequest, "ts_launch_shape": LaunchShape, "ts_queue_stats": QueueStats,
+    "ts_archive_info": ArchiveInfo, "ts_region_request": RegionRequest, "ts_server_options": ServerOptions,
+    "ts_server_stats": ServerStats,
 }
 
 _u32, _u64, _vp, _int = C.c_uint32, C.c_uint64, C.c_void_p, C.c_int
@@ -181,6 +202,18 @@ PROTOTYPES = {
     "ts_queue_launch": (_u32, [_vp, _vp]),
     "ts_queue_autotune": (_u32, [_vp, MutableBytes, _vp, _u64, _vp, _P(LaunchShape), _u64, _u32,
                                  _P(C.c_float)]),
+    "ts_archive_set_create": (_u32, [_P(_vp)]),
+    "ts_archive_set_destroy": (None, [_vp]),
+    "ts_archive_set_add": (_u32, [_vp, ConstBytes, _vp, _P(_u32)]),
+    "ts_archive_set_commit": (_u32, [_vp, _vp]),
+    "ts_archive_set_size": (_u32, [_vp]),
+    "ts_archive_set_info": (_u32, [_vp, _u32, _P(ArchiveInfo)]),
+    "ts_server_default_options": (None, [_P(ServerOptions)]),
+    "ts_server_create": (_u32, [_vp, _P(ServerOptions), _P(_vp)]),
+    "ts_server_destroy": (None, [_vp]),
+    "ts_server_check": (_u32, [_vp, _P(RegionRequest)]),
+    "ts_server_replay": (_u32, [_vp, _P(RegionRequest), _P(_u64), _u64, _P(_u64), _P(_u32), _P(ServerStats)]),
+    "ts_server_run_batch": (_u32, [_vp, _P(RegionRequest), _u64, _P(_u32)]),
     "ts_hip_malloc": (_int, [_u64, _P(_vp)]),
     "ts_hip_free": (_int, [_vp]),
     "ts_hip_host_malloc": (_int, [_u64, _P(_vp)]),

@@ -15,10 +15,10 @@ from typing import Optional, Sequence
 import numpy as np
 
 from . import _native as N
-from ._native import (BatchItem, BatchRequest, Calibration, CompressedInput, ConstBytes,
+from ._native import (ArchiveInfo, BatchItem, BatchRequest, Calibration, CompressedInput, ConstBytes,
                       ConstImageView, DeviceResult, ImageSpec, ImageView, LaunchShape, MutableBytes,
-                      Parameter, ParseLimits, QueueStats, Roi, SegmentMetadata, TileMapping,
-                      TileMetadata, TileShape, WorkspaceSpec)
+                      Parameter, ParseLimits, QueueStats, RegionRequest, Roi, SegmentMetadata,
+                      ServerOptions, ServerStats, TileMapping, TileMetadata, TileShape, WorkspaceSpec)
 
 MAX_BYTES = (1 << 63) - 1
 
@@ -468,6 +468,118 @@ class DecodeQueue:
     def close(self) -> None:
         if self.handle:
             N.lib().ts_queue_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+# ---- serving resident archives ----------------------------------------------------
+
+
+class ArchiveSet:
+    """Archives resident in device memory with their tile tables (tilestream/serve.h)."""
+
+    def __init__(self):
+        handle = C.c_void_p(0)
+        st = Status(N.lib().ts_archive_set_create(C.byref(handle)))
+        if st != Status.success:
+            raise RuntimeError(f"ts_archive_set_create: {st.name}")
+        self.handle = handle.value
+        self._keep = []
+
+    def add(self, device_file: DeviceMemory, metadata: Metadata, size: Optional[int] = None):
+        """Adds an archive already in device memory; returns (status, id)."""
+        ident = C.c_uint32(0)
+        n = device_file.size if size is None else size
+        st = Status(N.lib().ts_archive_set_add(self.handle, ConstBytes(device_file.ptr, n), metadata.handle,
+                                               C.byref(ident)))
+        if st == Status.success:
+            self._keep.append((device_file, metadata))
+        return st, ident.value
+
+    def commit(self, stream=None) -> Status:
+        return Status(N.lib().ts_archive_set_commit(self.handle, _stream(stream)))
+
+    def __len__(self) -> int:
+        return int(N.lib().ts_archive_set_size(self.handle))
+
+    def info(self, ident: int):
+        out = ArchiveInfo()
+        st = Status(N.lib().ts_archive_set_info(self.handle, ident, C.byref(out)))
+        return st, out
+
+    def close(self) -> None:
+        if self.handle:
+            N.lib().ts_archive_set_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def server_options(streams=None, max_batch=None, max_tasks=None, admission_bytes=None, shape=None) -> ServerOptions:
+    """The library's default server options with the given fields replaced."""
+    o = ServerOptions()
+    N.lib().ts_server_default_options(C.byref(o))
+    for name, value in (("streams", streams), ("max_batch", max_batch), ("max_tasks", max_tasks),
+                        ("admission_bytes", admission_bytes)):
+        if value is not None:
+            setattr(o, name, value)
+    if shape is not None:
+        o.shape = LaunchShape(*shape)
+    return o
+
+
+def region_request(archive: int, roi, output: ImageView, calibrated=False, offset=0.0, gain=1.0) -> RegionRequest:
+    return RegionRequest(archive, 1 if calibrated else 0, Roi(*roi), output, offset, gain)
+
+
+class Server:
+    """Continuous batching of region requests over resident archives."""
+
+    def __init__(self, archives: ArchiveSet, options: Optional[ServerOptions] = None):
+        handle = C.c_void_p(0)
+        opts = options or server_options()
+        st = Status(N.lib().ts_server_create(archives.handle, C.byref(opts), C.byref(handle)))
+        if st != Status.success:
+            raise RuntimeError(f"ts_server_create: {st.name}")
+        self.handle = handle.value
+        self._archives = archives
+
+    def check(self, request: RegionRequest) -> Status:
+        return Status(N.lib().ts_server_check(self.handle, C.byref(request)))
+
+    def replay(self, requests: Sequence[RegionRequest], arrival_ns):
+        """Open-loop replay; returns (status, latency_ns, statuses, stats)."""
+        n = len(requests)
+        arr = (RegionRequest * max(1, n))(*requests)
+        arrivals = np.ascontiguousarray(arrival_ns, dtype=np.uint64)
+        assert arrivals.size == n
+        latency = np.zeros(max(1, n), np.uint64)
+        statuses = np.zeros(max(1, n), np.uint32)
+        stats = ServerStats()
+        st = Status(N.lib().ts_server_replay(self.handle, arr, arrivals.ctypes.data_as(C.POINTER(C.c_uint64)), n,
+                                             latency.ctypes.data_as(C.POINTER(C.c_uint64)),
+                                             statuses.ctypes.data_as(C.POINTER(C.c_uint32)), C.byref(stats)))
+        return st, latency[:n], statuses[:n], stats
+
+    def run_batch(self, requests: Sequence[RegionRequest]):
+        n = len(requests)
+        arr = (RegionRequest * max(1, n))(*requests)
+        statuses = np.zeros(max(1, n), np.uint32)
+        st = Status(N.lib().ts_server_run_batch(self.handle, arr, n, statuses.ctypes.data_as(C.POINTER(C.c_uint32))))
+        return st, statuses[:n]
+
+    def close(self) -> None:
+        if self.handle:
+            N.lib().ts_server_destroy(self.handle)
             self.handle = None
 
     def __del__(self):

@@ -28,16 +28,19 @@ COMMON = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-fno-fast-math",
 HIP_FLAGS = ["-x", "hip", "--offload-arch=" + ARCH, "-fno-gpu-rdc",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 
+# Host-only sources: plain C++ against the HIP runtime API, no device pass.
+HOST_FLAGS = ["-x", "c++", "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include", "-Wall", "-Wextra"]
+
 HIP_SOURCES = ["hip/decode_mixed.hip", "hip/decode_u16.hip", "hip/decode_f32.hip", "hip/decode_u16_cal.hip",
                "hip/decode_f32_cal.hip", "hip/decode.hip", "hip/encode.hip", "hip/calibrate.hip"]
 HOST_SOURCES = ["host/sizing.cpp", "host/parser.cpp", "host/cpu_codec.cpp", "host/plan.cpp",
-                "host/queue.cpp", "host/capi.cpp"]
+                "host/queue.cpp", "host/serve.cpp", "host/capi.cpp"]
 
 
 def _compile(src: str) -> str:
     path = os.path.join(CSRC, src)
     obj = os.path.join(OBJ, src.replace("/", "_") + ".o")
-    flags = COMMON + (HIP_FLAGS if src.endswith(".hip") else ["-Wall", "-Wextra"])
+    flags = COMMON + (HIP_FLAGS if src.endswith(".hip") else HOST_FLAGS)
     cmd = [HIPCC] + flags + ["-c", path, "-o", obj]
     res = subprocess.run(cmd, capture_output=True, text=True)
     noise = "is not a recognized feature for this target"

@@ -308,12 +308,41 @@ struct Pending {
     gptr<const u32x4> blocks;
 };
 
+// Request mode: the request holding task i, found by a binary search between
+// the sampled index entries around i (at most TSK_REQUEST_STRIDE requests, all
+// loads scalar), and the tile of its region.
+__device__ inline tsk_decode_task request_task(const tsk_decode_args &a, std::uint64_t i, std::uint32_t &request) {
+    const std::uint64_t slot = i / TSK_REQUEST_STRIDE;
+    std::uint32_t r = load_constant(a.request_index, slot);
+    std::uint32_t hi = (slot + 1) * TSK_REQUEST_STRIDE < a.task_count ? load_constant(a.request_index, slot + 1)
+                                                                      : a.request_count - 1;
+    // Invariant: requests[r].first_task <= i < requests[hi + 1].first_task.
+    while (r < hi) {
+        const std::uint32_t mid = r + (hi - r + 1) / 2;
+        if (load_constant(a.requests, mid).first_task <= i)
+            r = mid;
+        else
+            hi = mid - 1;
+    }
+    const tsk_request q = load_constant(a.requests, r);
+    const std::uint32_t local = static_cast<std::uint32_t>(i - q.first_task);
+    const std::uint32_t dr = local / q.cols;
+    const std::uint32_t col = q.col0 + (local - dr * q.cols);
+    const std::uint32_t row = q.row0 + dr;
+    tsk_decode_task t = load_constant(a.tasks, std::uint64_t{q.tile_base} + std::uint64_t{row} * q.tile_columns + col);
+    t.column = col;
+    t.row = row;
+    request = r;
+    return t;
+}
+
 // Reads a task descriptor and, on the trusted path, the producer's result and
 // the device-resident index entry (reference decoder.cu:191-220).
 __device__ inline Pending describe(const tsk_decode_args &a, std::uint64_t i) {
-    const tsk_decode_task task = load_constant(a.tasks, i);
+    std::uint32_t request = 0;
+    const tsk_decode_task task = a.request_count != 0 ? request_task(a, i, request) : load_constant(a.tasks, i);
     Pending p;
-    p.image = task.image;
+    p.image = a.request_count != 0 ? request : task.image;
     p.column = task.column;
     p.row = task.row;
     p.rows = task.rows;

@@ -6,6 +6,7 @@
 #include <tilestream/batch.h>
 #include <tilestream/cpu.h>
 #include <tilestream/hip.h>
+#include <tilestream/serve.h>
 #include <tilestream_c.h>
 
 #include <cstddef>
@@ -54,6 +55,24 @@ struct ts_queue {
 
 TS_SAME_LAYOUT(ts_launch_shape, hip::LaunchShape);
 TS_SAME_LAYOUT(ts_queue_stats, hip::QueueStats);
+
+struct ts_archive_set {
+    std::unique_ptr<hip::ArchiveSet> set;
+};
+
+struct ts_server {
+    std::unique_ptr<hip::Server> server;
+};
+
+TS_SAME_LAYOUT(ts_archive_info, hip::ArchiveInfo);
+TS_SAME_LAYOUT(ts_region_request, hip::RegionRequest);
+TS_SAME_LAYOUT(ts_server_options, hip::ServerOptions);
+TS_SAME_LAYOUT(ts_server_stats, hip::ServerStats);
+static_assert(offsetof(ts_region_request, output) == offsetof(hip::RegionRequest, output));
+static_assert(offsetof(ts_region_request, offset) == offsetof(hip::RegionRequest, offset));
+static_assert(offsetof(ts_server_options, admission_bytes) == offsetof(hip::ServerOptions, admission_bytes));
+static_assert(offsetof(ts_server_options, shape) == offsetof(hip::ServerOptions, shape));
+static_assert(offsetof(ts_archive_info, tile_count) == offsetof(hip::ArchiveInfo, tile_count));
 
 namespace {
 
@@ -275,6 +294,69 @@ uint32_t ts_queue_autotune(ts_queue *q, ts_mutable_bytes ws, ts_device_result *r
     }
 }
 
+uint32_t ts_archive_set_create(ts_archive_set **set) {
+    if (set == nullptr) return code(Status::invalid_argument);
+    try {
+        std::unique_ptr<hip::ArchiveSet> fresh;
+        if (const Status st = hip::ArchiveSet::create(fresh); st != Status::success) return code(st);
+        auto *handle = new ts_archive_set{std::move(fresh)};
+        delete *set;
+        *set = handle;
+        return 0;
+    } catch (const std::bad_alloc &) {
+        return code(Status::resource_limit);
+    }
+}
+void ts_archive_set_destroy(ts_archive_set *set) { delete set; }
+uint32_t ts_archive_set_add(ts_archive_set *set, ts_const_bytes file, const ts_metadata *metadata, uint32_t *id) {
+    if (set == nullptr || metadata == nullptr || id == nullptr) return code(Status::invalid_argument);
+    return code(set->set->add(as<ConstBytes>(file), *meta(metadata), *id));
+}
+uint32_t ts_archive_set_commit(ts_archive_set *set, void *stream) {
+    if (set == nullptr) return code(Status::invalid_argument);
+    return code(set->set->commit(stream_of(stream)));
+}
+uint32_t ts_archive_set_size(const ts_archive_set *set) { return set->set->size(); }
+uint32_t ts_archive_set_info(const ts_archive_set *set, uint32_t id, ts_archive_info *out) {
+    if (set == nullptr || out == nullptr) return code(Status::invalid_argument);
+    return code(set->set->info(id, as<hip::ArchiveInfo>(*out)));
+}
+void ts_server_default_options(ts_server_options *out) {
+    const hip::ServerOptions o;
+    std::memcpy(out, &o, sizeof o);
+}
+uint32_t ts_server_create(const ts_archive_set *set, const ts_server_options *options, ts_server **server) {
+    if (set == nullptr || options == nullptr || server == nullptr) return code(Status::invalid_argument);
+    try {
+        std::unique_ptr<hip::Server> fresh;
+        if (const Status st = hip::Server::create(*set->set, as<hip::ServerOptions>(*options), fresh);
+            st != Status::success)
+            return code(st);
+        auto *handle = new ts_server{std::move(fresh)};
+        delete *server;
+        *server = handle;
+        return 0;
+    } catch (const std::bad_alloc &) {
+        return code(Status::resource_limit);
+    }
+}
+void ts_server_destroy(ts_server *server) { delete server; }
+uint32_t ts_server_check(const ts_server *server, const ts_region_request *request) {
+    if (server == nullptr || request == nullptr) return code(Status::invalid_argument);
+    return code(server->server->check(as<hip::RegionRequest>(*request)));
+}
+uint32_t ts_server_replay(ts_server *server, const ts_region_request *requests, const uint64_t *arrival_ns,
+                          uint64_t count, uint64_t *latency_ns, uint32_t *statuses, ts_server_stats *stats) {
+    if (server == nullptr || stats == nullptr) return code(Status::invalid_argument);
+    return code(server->server->replay(reinterpret_cast<const hip::RegionRequest *>(requests), arrival_ns, count,
+                                       latency_ns, statuses, as<hip::ServerStats>(*stats)));
+}
+uint32_t ts_server_run_batch(ts_server *server, const ts_region_request *requests, uint64_t count,
+                             uint32_t *statuses) {
+    if (server == nullptr) return code(Status::invalid_argument);
+    return code(server->server->run_batch(reinterpret_cast<const hip::RegionRequest *>(requests), count, statuses));
+}
+
 int ts_hip_malloc(uint64_t bytes, void **data) { return hipMalloc(data, bytes); }
 int ts_hip_free(void *data) { return hipFree(data); }
 int ts_hip_host_malloc(uint64_t bytes, void **data) { return hipHostMalloc(data, bytes, hipHostMallocMapped); }
@@ -336,6 +418,10 @@ uint64_t ts_abi_sizeof(const char *name) {
     TS_SIZE(ts_batch_request)
     TS_SIZE(ts_launch_shape)
     TS_SIZE(ts_queue_stats)
+    TS_SIZE(ts_archive_info)
+    TS_SIZE(ts_region_request)
+    TS_SIZE(ts_server_options)
+    TS_SIZE(ts_server_stats)
 #undef TS_SIZE
     return 0;
 }

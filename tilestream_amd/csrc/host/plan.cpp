@@ -119,6 +119,8 @@ Status check_calibration(const Calibration &c, const ImageSpec &region, std::vec
     return st != Status::success ? st : check_parameter(c.gain, region, readable, pointers);
 }
 
+} // namespace
+
 // Repeats the parser's structural checks that keep device reads inside the tile
 // payload (reference plan.cpp:238-279).
 Status check_tile(const ImageMetadata &m, std::uint64_t index) noexcept {
@@ -155,6 +157,8 @@ Status check_tile(const ImageMetadata &m, std::uint64_t index) noexcept {
     }
     return Status::success;
 }
+
+namespace {
 
 // Selects the tiles intersecting the region, rebased into the supplied bytes
 // when the payloads were staged (reference plan.cpp:282-349).

@@ -94,6 +94,7 @@ inline Status hip_failure() noexcept {
 inline Status hip_status(int error) noexcept { return error == hipSuccess ? Status::success : hip_failure(); }
 
 [[nodiscard]] Status accessible(const void *data, int device, bool pageable) noexcept;
+[[nodiscard]] Status check_tile(const ImageMetadata &metadata, std::uint64_t index) noexcept;
 [[nodiscard]] Status stream_device(hipStream_t stream, int device) noexcept;
 [[nodiscard]] Status check_execution(const PlanState *plan, Operation operation, MutableBytes workspace,
                                      ResultBuffer results, hipStream_t stream) noexcept;

@@ -1,0 +1,535 @@
+// Serving resident archives (tilestream/serve.h): archive tile tables planned
+// once, region batches described by a few bytes per request, continuous
+// batching over several streams.
+#include "plan.h"
+
+#include <tilestream/format.h>
+#include <tilestream/serve.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <new>
+#include <stdexcept>
+#include <thread>
+
+namespace tilestream::hip {
+
+namespace {
+
+struct Archive {
+    ArchiveInfo info;
+    const std::uint8_t *bytes = nullptr; // device
+    std::uint64_t tile_base = 0;         // first entry in the set's tile table
+    std::uint32_t tile_columns = 0;
+};
+
+constexpr std::uint64_t align_up(std::uint64_t v, std::uint64_t a) { return (v + a - 1) / a * a; }
+
+} // namespace
+
+struct ArchiveSet::Impl {
+    int device = 0;
+    bool pageable = false;
+    std::vector<Archive> archives;
+    std::vector<tsk_decode_task> tiles;
+    std::vector<std::uint16_t> rows;
+    // committed tables
+    void *storage = nullptr;
+    const tsk_decode_task *device_tiles = nullptr;
+    const std::uint16_t *device_rows = nullptr;
+    std::uint64_t committed = 0;
+    std::uint64_t row_count = 0;
+
+    ~Impl() {
+        if (storage != nullptr) static_cast<void>(hipFree(storage));
+    }
+};
+
+ArchiveSet::ArchiveSet(std::unique_ptr<Impl> impl) noexcept : impl_(std::move(impl)) {}
+ArchiveSet::~ArchiveSet() = default;
+
+Status ArchiveSet::create(std::unique_ptr<ArchiveSet> &set) noexcept {
+    try {
+        auto impl = std::make_unique<Impl>();
+        int pageable = 0;
+        if (hipGetDevice(&impl->device) != hipSuccess ||
+            hipDeviceGetAttribute(&pageable, hipDeviceAttributePageableMemoryAccess, impl->device) != hipSuccess)
+            return detail::hip_failure();
+        impl->pageable = pageable != 0;
+        set.reset(new ArchiveSet(std::move(impl)));
+        return Status::success;
+    } catch (const std::bad_alloc &) {
+        return Status::resource_limit;
+    }
+}
+
+// Every tile of the archive is checked as an ROI decode checks the tiles it
+// selects (reference plan.cpp:238-279), then entered into the tile table with
+// its absolute payload address and, for predictive tiles, its row offsets.
+Status ArchiveSet::add(ConstBytes device_file, const ImageMetadata &m, std::uint32_t &id) noexcept {
+    Impl &s = *impl_;
+    if (s.archives.size() >= UINT32_MAX) return Status::resource_limit;
+    std::uint64_t bound = 0;
+    if (const Status st = compress_bound(m.image, m.tile, bound); st != Status::success) return st;
+    if (const Status st = host::byte_range(device_file.data, device_file.size); st != Status::success) return st;
+    if (device_file.data == nullptr || reinterpret_cast<std::uintptr_t>(device_file.data) % 4 != 0)
+        return Status::invalid_argument;
+    if (m.file_bytes > device_file.size) return Status::buffer_too_small;
+    const std::uint64_t across = host::ceil_div(m.image.width, m.tile.width);
+    const std::uint64_t down = host::ceil_div(m.image.height, m.tile.height);
+    if (m.tiles.size() != across * down) return Status::malformed_data;
+    if (const Status st = detail::accessible(device_file.data, s.device, s.pageable); st != Status::success) return st;
+    try {
+        const std::size_t tiles_before = s.tiles.size(), rows_before = s.rows.size();
+        auto rollback = [&](Status st) {
+            s.tiles.resize(tiles_before);
+            s.rows.resize(rows_before);
+            return st;
+        };
+        if (tiles_before + m.tiles.size() > UINT32_MAX) return Status::resource_limit;
+        const auto *base = static_cast<const std::uint8_t *>(device_file.data);
+        for (std::uint64_t index = 0; index < m.tiles.size(); ++index) {
+            if (const Status st = detail::check_tile(m, index); st != Status::success) return rollback(st);
+            const TileMetadata &t = m.tiles[static_cast<std::size_t>(index)];
+            tsk_decode_task task{};
+            task.payload = base + t.offset;
+            task.column = static_cast<std::uint32_t>(index % across);
+            task.row = static_cast<std::uint32_t>(index / across);
+            task.image = 0; // request mode: the request is the image
+            task.rows = TSK_NO_ROWS;
+            task.words = static_cast<std::uint32_t>(t.length / 4);
+            task.shape = TSK_TASK_SHAPE(static_cast<std::uint32_t>(t.mode), t.height, t.width);
+            if (t.mode == TileMode::predict_bitpack) {
+                if (s.rows.size() + t.height > UINT32_MAX) return rollback(Status::resource_limit);
+                task.rows = static_cast<std::uint32_t>(s.rows.size());
+                for (std::uint64_t r = 0; r < t.height; ++r) {
+                    const SegmentMetadata &g = m.segments[static_cast<std::size_t>(t.first_segment + r)];
+                    s.rows.push_back(static_cast<std::uint16_t>((g.payload_offset - format::segment_header_bytes) / 4));
+                }
+            }
+            s.tiles.push_back(task);
+        }
+        Archive a;
+        a.info.image = m.image;
+        a.info.tile = m.tile;
+        a.info.tile_count = m.tiles.size();
+        a.info.file_bytes = m.file_bytes;
+        a.bytes = base;
+        a.tile_base = tiles_before;
+        a.tile_columns = static_cast<std::uint32_t>(across);
+        s.archives.push_back(a);
+        id = static_cast<std::uint32_t>(s.archives.size() - 1);
+        return Status::success;
+    } catch (const std::bad_alloc &) {
+        return Status::resource_limit;
+    } catch (const std::length_error &) {
+        return Status::resource_limit;
+    }
+}
+
+Status ArchiveSet::commit(hipStream_t stream) noexcept {
+    Impl &s = *impl_;
+    if (const Status st = detail::stream_device(stream, s.device); st != Status::success) return st;
+    const std::uint64_t rows_at = align_up(s.tiles.size() * sizeof(tsk_decode_task), 256);
+    const std::uint64_t total = rows_at + s.rows.size() * sizeof(std::uint16_t) + 256;
+    void *storage = nullptr;
+    if (hipMalloc(&storage, total) != hipSuccess) return detail::hip_failure();
+    auto *base = static_cast<std::uint8_t *>(storage);
+    if ((!s.tiles.empty() && hipMemcpyAsync(base, s.tiles.data(), s.tiles.size() * sizeof(tsk_decode_task),
+                                            hipMemcpyHostToDevice, stream) != hipSuccess) ||
+        (!s.rows.empty() && hipMemcpyAsync(base + rows_at, s.rows.data(), s.rows.size() * sizeof(std::uint16_t),
+                                           hipMemcpyHostToDevice, stream) != hipSuccess) ||
+        hipStreamSynchronize(stream) != hipSuccess) {
+        static_cast<void>(hipFree(storage));
+        return detail::hip_failure();
+    }
+    if (s.storage != nullptr) static_cast<void>(hipFree(s.storage));
+    s.storage = storage;
+    s.device_tiles = reinterpret_cast<const tsk_decode_task *>(base);
+    s.device_rows = reinterpret_cast<const std::uint16_t *>(base + rows_at);
+    s.committed = s.archives.size();
+    s.row_count = s.rows.size();
+    return Status::success;
+}
+
+std::uint32_t ArchiveSet::size() const noexcept { return static_cast<std::uint32_t>(impl_->archives.size()); }
+
+Status ArchiveSet::info(std::uint32_t id, ArchiveInfo &out) const noexcept {
+    if (id >= impl_->archives.size()) return Status::invalid_argument;
+    out = impl_->archives[id].info;
+    return Status::success;
+}
+
+// ---------------------------------------------------------------------------
+
+namespace {
+
+// One batch in flight: a stream, the batch's tables staged in pinned host
+// memory and uploaded with one copy, and the event that marks its end.
+//
+// Table layout (host and device alike): work counter | DeviceResult[n] |
+// tsk_decode_image[n] | tsk_request[n] | request index.
+struct Slot {
+    hipStream_t stream = nullptr;
+    hipEvent_t done = nullptr;
+    std::uint8_t *host = nullptr;
+    std::uint8_t *device = nullptr;
+    std::uint64_t first = 0, count = 0; // requests [first, first + count) of the current replay
+    std::uint64_t bytes = 0;            // output bytes admitted
+    bool busy = false;
+};
+
+constexpr std::uint64_t kCounterBytes = 256;
+
+struct Layout {
+    std::uint64_t results, images, requests, index, total;
+};
+
+Layout layout_for(std::uint64_t requests, std::uint64_t tasks) {
+    Layout l{};
+    l.results = kCounterBytes;
+    l.images = align_up(l.results + requests * sizeof(DeviceResult), 256);
+    l.requests = align_up(l.images + requests * sizeof(tsk_decode_image), 256);
+    l.index = align_up(l.requests + requests * sizeof(tsk_request), 256);
+    l.total = l.index + (tasks / TSK_REQUEST_STRIDE + 1) * sizeof(std::uint32_t);
+    return l;
+}
+
+// The tiles a request decodes, and its output bytes.
+struct Extent {
+    std::uint32_t col0 = 0, row0 = 0, cols = 0, rows = 0;
+    std::uint64_t tasks = 0;
+    std::uint64_t bytes = 0;   // output bytes
+    std::uint64_t decoded = 0; // region pixels x stored sample bytes
+};
+
+} // namespace
+
+struct Server::Impl {
+    const ArchiveSet::Impl *set = nullptr;
+    ServerOptions options;
+    int device = 0;
+    int cus = 0;
+    int per_cu[5] = {};
+    std::vector<Slot> slots;
+    std::vector<Extent> extents; // per request of the current replay
+
+    ~Impl() {
+        for (Slot &s : slots) {
+            if (s.stream != nullptr) static_cast<void>(hipStreamSynchronize(s.stream));
+            if (s.done != nullptr) static_cast<void>(hipEventDestroy(s.done));
+            if (s.stream != nullptr) static_cast<void>(hipStreamDestroy(s.stream));
+            if (s.host != nullptr) static_cast<void>(hipHostFree(s.host));
+            if (s.device != nullptr) static_cast<void>(hipFree(s.device));
+        }
+    }
+
+    Status extent(const RegionRequest &r, Extent &e) const noexcept;
+    Status launch(Slot &slot, const RegionRequest *requests, std::uint64_t first, std::uint64_t count,
+                  std::uint64_t tasks) noexcept;
+};
+
+Server::Server(std::unique_ptr<Impl> impl) noexcept : impl_(std::move(impl)) {}
+Server::~Server() = default;
+
+Status Server::create(const ArchiveSet &archives, const ServerOptions &options, std::unique_ptr<Server> &server) noexcept {
+    const ArchiveSet::Impl *set = archives.impl();
+    if (set == nullptr || set->committed == 0 || set->committed != set->archives.size()) return Status::invalid_argument;
+    if (options.streams == 0 || options.streams > 16 || options.max_batch == 0 || options.max_tasks == 0 ||
+        options.admission_bytes == 0)
+        return Status::invalid_argument;
+    const LaunchShape &sh = options.shape;
+    if (!tsk_decode_supported(sh.group, sh.block, sh.depth)) return Status::invalid_argument;
+    try {
+        auto impl = std::make_unique<Impl>();
+        impl->set = set;
+        impl->options = options;
+        if (hipGetDevice(&impl->device) != hipSuccess || impl->device != set->device ||
+            hipDeviceGetAttribute(&impl->cus, hipDeviceAttributeMultiprocessorCount, impl->device) != hipSuccess)
+            return detail::hip_failure();
+        for (std::uint32_t kind = 0; kind < 5; ++kind)
+            if (tsk_decode_occupancy(kind, sh.group, sh.block, sh.depth, &impl->per_cu[kind]) != hipSuccess)
+                return detail::hip_failure();
+        if (tsk_preload(1) != hipSuccess || tsk_preload(3) != hipSuccess) return detail::hip_failure();
+        const Layout cap = layout_for(options.max_batch, options.max_tasks);
+        impl->slots.resize(options.streams);
+        for (Slot &s : impl->slots) {
+            if (hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking) != hipSuccess) {
+                s.stream = nullptr;
+                return detail::hip_failure();
+            }
+            if (hipEventCreateWithFlags(&s.done, hipEventDisableTiming) != hipSuccess) {
+                s.done = nullptr;
+                return detail::hip_failure();
+            }
+            void *h = nullptr, *d = nullptr;
+            if (hipHostMalloc(&h, cap.total, hipHostMallocDefault) != hipSuccess) return detail::hip_failure();
+            s.host = static_cast<std::uint8_t *>(h);
+            if (hipMalloc(&d, cap.total) != hipSuccess) return detail::hip_failure();
+            s.device = static_cast<std::uint8_t *>(d);
+        }
+        server.reset(new Server(std::move(impl)));
+        return Status::success;
+    } catch (const std::bad_alloc &) {
+        return Status::resource_limit;
+    } catch (const std::length_error &) {
+        return Status::resource_limit;
+    }
+}
+
+Status Server::Impl::extent(const RegionRequest &r, Extent &e) const noexcept {
+    if (r.archive >= set->committed || r.calibrated > 1) return Status::invalid_argument;
+    const Archive &a = set->archives[r.archive];
+    const ImageSpec &image = a.info.image;
+    const Roi &roi = r.roi;
+    if (roi.width == 0 || roi.height == 0 || std::uint64_t{roi.x} + roi.width > image.width ||
+        std::uint64_t{roi.y} + roi.height > image.height)
+        return Status::invalid_roi;
+    if (const Status st = host::check_view(host::as_const(r.output)); st != Status::success) return st;
+    if (r.output.image.type != (r.calibrated ? PixelType::float32 : image.type) || r.output.image.width != roi.width ||
+        r.output.image.height != roi.height)
+        return Status::invalid_argument;
+    const TileShape &t = a.info.tile;
+    e.col0 = roi.x / t.width;
+    e.row0 = roi.y / t.height;
+    e.cols = (roi.x + roi.width - 1) / t.width - e.col0 + 1;
+    e.rows = (roi.y + roi.height - 1) / t.height - e.row0 + 1;
+    e.tasks = std::uint64_t{e.cols} * e.rows;
+    if (e.tasks > options.max_tasks) return Status::resource_limit;
+    const std::uint64_t area = std::uint64_t{roi.width} * roi.height;
+    const std::uint64_t stored = image.type == PixelType::uint16 ? 2 : 4;
+    e.decoded = area * stored;
+    e.bytes = area * (r.calibrated ? 4 : stored);
+    return Status::success;
+}
+
+Status Server::check(const RegionRequest &request) const noexcept {
+    Extent e;
+    return impl_->extent(request, e);
+}
+
+// Stages the batch's tables, uploads them with one copy, decodes in request
+// mode, completes the results and copies them back, then marks the end.
+Status Server::Impl::launch(Slot &slot, const RegionRequest *requests, std::uint64_t first, std::uint64_t count,
+                            std::uint64_t tasks) noexcept {
+    const Layout l = layout_for(count, tasks);
+    std::uint8_t *h = slot.host;
+    std::memset(h, 0, l.images); // work counter and results start at zero
+    auto *images = reinterpret_cast<tsk_decode_image *>(h + l.images);
+    auto *reqs = reinterpret_cast<tsk_request *>(h + l.requests);
+    auto *index = reinterpret_cast<std::uint32_t *>(h + l.index);
+    bool u16 = false, f32 = false, cal = false, plain = false;
+    std::uint64_t task = 0;
+    for (std::uint64_t k = 0; k < count; ++k) {
+        const RegionRequest &r = requests[first + k];
+        const Extent &e = extents[first + k];
+        const Archive &a = set->archives[r.archive];
+        tsk_decode_image &im = images[k];
+        im = tsk_decode_image{};
+        im.src = a.bytes;
+        im.out = static_cast<std::uint8_t *>(r.output.data);
+        im.src_size = a.info.file_bytes;
+        im.out_pitch = r.output.pitch_bytes;
+        im.width = a.info.image.width;
+        im.height = a.info.image.height;
+        im.roi_x = r.roi.x;
+        im.roi_y = r.roi.y;
+        im.roi_w = r.roi.width;
+        im.roi_h = r.roi.height;
+        im.tile_columns = a.tile_columns;
+        im.tile_w = a.info.tile.width;
+        im.tile_h = a.info.tile.height;
+        im.type = static_cast<std::uint8_t>(a.info.image.type);
+        im.calibrated = static_cast<std::uint8_t>(r.calibrated);
+        im.offset = r.calibrated ? r.offset : 0.0F;
+        im.gain = r.calibrated ? r.gain : 1.0F;
+        tsk_request &q = reqs[k];
+        q.first_task = task;
+        q.tile_base = static_cast<std::uint32_t>(a.tile_base);
+        q.tile_columns = a.tile_columns;
+        q.col0 = e.col0;
+        q.row0 = e.row0;
+        q.cols = e.cols;
+        q.reserved = 0;
+        // Every TSK_REQUEST_STRIDE-th task names the request that holds it.
+        for (std::uint64_t s = (task + TSK_REQUEST_STRIDE - 1) / TSK_REQUEST_STRIDE * TSK_REQUEST_STRIDE;
+             s < task + e.tasks; s += TSK_REQUEST_STRIDE)
+            index[s / TSK_REQUEST_STRIDE] = static_cast<std::uint32_t>(k);
+        task += e.tasks;
+        u16 = u16 || a.info.image.type == PixelType::uint16;
+        f32 = f32 || a.info.image.type == PixelType::float32;
+        cal = cal || r.calibrated != 0;
+        plain = plain || r.calibrated == 0;
+    }
+    std::uint32_t kind = TSK_KIND_MIXED;
+    if (u16 != f32 && cal != plain)
+        kind = u16 ? (cal ? TSK_KIND_U16_CAL : TSK_KIND_U16) : (cal ? TSK_KIND_F32_CAL : TSK_KIND_F32);
+
+    const LaunchShape &sh = options.shape;
+    std::uint64_t resident = static_cast<std::uint64_t>(cus) * static_cast<std::uint64_t>(std::max(1, per_cu[kind]));
+    std::uint64_t grid = sh.grid != 0 ? sh.grid : std::min<std::uint64_t>(resident, tasks);
+    const bool queued = sh.chunk != 0 && tasks > grid;
+
+    tsk_decode_args args{};
+    args.images = reinterpret_cast<const tsk_decode_image *>(slot.device + l.images);
+    args.tasks = set->device_tiles;
+    args.rows = set->device_rows;
+    args.results = slot.device + l.results;
+    args.queue = queued ? reinterpret_cast<std::uint64_t *>(slot.device) : nullptr;
+    args.task_count = tasks;
+    args.row_count = set->row_count;
+    args.image_count = static_cast<std::uint32_t>(count);
+    args.kind = kind;
+    args.group = sh.group;
+    args.block = sh.block;
+    args.grid = static_cast<std::uint32_t>(std::max<std::uint64_t>(1, grid));
+    args.chunk = queued ? sh.chunk : 0;
+    args.depth = sh.depth;
+    args.request_count = static_cast<std::uint32_t>(count);
+    args.requests = reinterpret_cast<const tsk_request *>(slot.device + l.requests);
+    args.request_index = reinterpret_cast<const std::uint32_t *>(slot.device + l.index);
+    if (hipMemcpyAsync(slot.device, h, l.total, hipMemcpyHostToDevice, slot.stream) != hipSuccess ||
+        tsk_decode(&args, slot.stream) != hipSuccess ||
+        tsk_results_complete(args.images, static_cast<std::uint32_t>(count), slot.device + l.results, slot.stream) !=
+            hipSuccess ||
+        hipMemcpyAsync(h + l.results, slot.device + l.results, count * sizeof(DeviceResult), hipMemcpyDeviceToHost,
+                       slot.stream) != hipSuccess ||
+        hipEventRecord(slot.done, slot.stream) != hipSuccess)
+        return detail::hip_failure();
+    slot.first = first;
+    slot.count = count;
+    slot.busy = true;
+    return Status::success;
+}
+
+Status Server::replay(const RegionRequest *requests, const std::uint64_t *arrival_ns, std::uint64_t count,
+                      std::uint64_t *latency_ns, std::uint32_t *statuses, ServerStats &stats) noexcept {
+    Impl &s = *impl_;
+    stats = ServerStats{};
+    if (count == 0) return Status::success;
+    if (requests == nullptr || arrival_ns == nullptr) return Status::invalid_argument;
+    int current = 0;
+    if (hipGetDevice(&current) != hipSuccess) return detail::hip_failure();
+    if (current != s.device) return Status::invalid_argument;
+    try {
+        // Validate everything before the clock starts.
+        s.extents.assign(count, Extent{});
+        std::vector<detail::Range> writable, readable;
+        writable.reserve(count);
+        for (std::uint64_t i = 0; i < count; ++i) {
+            if (i > 0 && arrival_ns[i] < arrival_ns[i - 1]) return Status::invalid_argument;
+            if (const Status st = s.extent(requests[i], s.extents[i]); st != Status::success) return st;
+            const RegionRequest &r = requests[i];
+            std::uint64_t span = 0;
+            if (const Status st = image_span(r.output.image, r.output.pitch_bytes, span); st != Status::success)
+                return st;
+            if (const Status st = detail::accessible(r.output.data, s.device, s.set->pageable); st != Status::success)
+                return st;
+            const auto at = reinterpret_cast<std::uintptr_t>(r.output.data);
+            writable.push_back({at, at + span});
+        }
+        for (const Archive &a : s.set->archives) {
+            const auto at = reinterpret_cast<std::uintptr_t>(a.bytes);
+            readable.push_back({at, at + a.info.file_bytes});
+        }
+        detail::RangeSet ranges;
+        if (!ranges.assign(std::move(writable), std::move(readable))) return Status::invalid_argument;
+
+        using clock = std::chrono::steady_clock;
+        const auto start = clock::now();
+        auto elapsed = [&] {
+            return static_cast<std::uint64_t>(std::chrono::duration_cast<std::chrono::nanoseconds>(clock::now() - start).count());
+        };
+        std::uint64_t arrived = 0, dispatched = 0, finished = 0, in_flight = 0;
+        std::uint64_t last_done = 0;
+        Status failure = Status::success;
+        auto drain = [&] {
+            for (Slot &slot : s.slots)
+                if (slot.busy) static_cast<void>(hipStreamSynchronize(slot.stream));
+        };
+        while (finished < count) {
+            const std::uint64_t now = elapsed();
+            while (arrived < count && arrival_ns[arrived] <= now) ++arrived;
+            bool progress = false;
+            for (Slot &slot : s.slots) {
+                if (!slot.busy) continue;
+                const hipError_t q = hipEventQuery(slot.done);
+                if (q == hipErrorNotReady) continue;
+                if (q != hipSuccess) {
+                    failure = detail::hip_failure();
+                    break;
+                }
+                const std::uint64_t t = elapsed();
+                const auto *res = reinterpret_cast<const DeviceResult *>(slot.host + kCounterBytes);
+                for (std::uint64_t k = 0; k < slot.count; ++k) {
+                    const std::uint64_t i = slot.first + k;
+                    if (latency_ns != nullptr) latency_ns[i] = t - arrival_ns[i];
+                    if (statuses != nullptr) statuses[i] = static_cast<std::uint32_t>(res[k].status);
+                    if (res[k].status != Status::success) ++stats.failed;
+                    stats.decoded_bytes += s.extents[i].decoded;
+                }
+                finished += slot.count;
+                in_flight -= slot.bytes;
+                last_done = t;
+                slot.busy = false;
+                progress = true;
+            }
+            if (failure != Status::success) break;
+            for (Slot &slot : s.slots) {
+                if (slot.busy || dispatched == arrived) continue;
+                // Admit arrived requests in order while the batch and the
+                // device have room; one request is always admitted alone.
+                std::uint64_t n = 0, tasks = 0, bytes = 0;
+                while (dispatched + n < arrived && n < s.options.max_batch) {
+                    const Extent &e = s.extents[dispatched + n];
+                    if (tasks + e.tasks > s.options.max_tasks) break;
+                    if (in_flight + bytes + e.bytes > s.options.admission_bytes && (n != 0 || in_flight != 0)) break;
+                    tasks += e.tasks;
+                    bytes += e.bytes;
+                    ++n;
+                }
+                if (n == 0) break;
+                if (const Status st = s.launch(slot, requests, dispatched, n, tasks); st != Status::success) {
+                    failure = st;
+                    break;
+                }
+                slot.bytes = bytes;
+                in_flight += bytes;
+                dispatched += n;
+                ++stats.batches;
+                progress = true;
+            }
+            if (failure != Status::success) break;
+            if (!progress && dispatched == arrived && arrived < count && in_flight == 0) {
+                // Idle until the next arrival.
+                const std::uint64_t wait = arrival_ns[arrived] - std::min(arrival_ns[arrived], elapsed());
+                if (wait > 200000) std::this_thread::sleep_for(std::chrono::nanoseconds(wait - 100000));
+            }
+        }
+        if (failure != Status::success) {
+            drain();
+            for (Slot &slot : s.slots) slot.busy = false;
+            return failure;
+        }
+        stats.requests = count;
+        stats.seconds = static_cast<double>(last_done - std::min(last_done, arrival_ns[0])) * 1e-9;
+        return Status::success;
+    } catch (const std::bad_alloc &) {
+        return Status::resource_limit;
+    } catch (const std::length_error &) {
+        return Status::resource_limit;
+    }
+}
+
+Status Server::run_batch(const RegionRequest *requests, std::uint64_t count, std::uint32_t *statuses) noexcept {
+    try {
+        std::vector<std::uint64_t> arrivals(count, 0);
+        ServerStats stats;
+        return replay(requests, arrivals.data(), count, nullptr, statuses, stats);
+    } catch (const std::bad_alloc &) {
+        return Status::resource_limit;
+    }
+}
+
+} // namespace tilestream::hip
