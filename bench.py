@@ -83,6 +83,129 @@ def cpu_baseline(batch, threads: int, single_budget_bytes: int):
     }
 
 
+def percentile(values, q: float) -> float:
+    """Nearest-rank percentile (the reference harness's convention)."""
+    v = np.sort(np.asarray(values))
+    k = max(1, int(np.ceil(q / 100.0 * len(v))))
+    return float(v[k - 1])
+
+
+def roi_leg(args, threads: int, stream):
+    """ROI batch over resident archives: 4096x4096 uint16 detector archives with
+    64x64 tiles, region requests with log-uniform sides in [32, 2048] at uniform
+    origins, each fused with scalar calibration (x - 100) * 1.05 to FP32.
+    Served by the continuous-batching server (request-mode decode)."""
+    from tilestream_amd import api, generators
+    from tilestream_amd.workloads import ResidentBatch
+
+    side, tile = 4096, (64, 64)
+    offset, gain = 100.0, 1.05
+    t0 = time.perf_counter()
+    layout = [(side, side, np.uint16, tile)] * args.roi_archives
+    rb = ResidentBatch.build(layout, args.seed + 4, threads)
+    aset = api.ArchiveSet()
+    for e in rb.entries:
+        st, _ = aset.add(rb.src, e.meta, len(e.archive), e.src_offset)
+        assert st == api.Status.success, st
+    assert aset.commit(stream) == api.Status.success
+    rois = generators.roi_requests(args.roi_requests, args.seed + 5, side, side, archives=args.roi_archives)
+    offsets, total = [], 0
+    for a, x, y, w, h in rois:
+        offsets.append(total)
+        total += (w * h * 4 + 255) // 256 * 256
+    out = api.DeviceMemory(total)
+    requests = []
+    for (a, x, y, w, h), off in zip(rois, offsets):
+        view = api.device_view(out, api.PixelType.float32, w, h, offset=off, capacity=w * h * 4)
+        requests.append(api.region_request(a, (x, y, w, h), view, True, offset, gain))
+    decoded = sum(w * h * 2 for _, _, _, w, h in rois)
+    log(f"roi: {len(rb.entries)} archives ({rb.archive_bytes / 1e9:.2f} GB), {len(rois)} requests, "
+        f"{decoded / 1e9:.2f} GB decoded, {total / 1e9:.2f} GB FP32 out, set up in {time.perf_counter() - t0:.1f} s")
+
+    server = api.Server(aset, api.server_options(streams=args.roi_streams, max_batch=args.roi_max_batch))
+    n = len(requests)
+    # Saturation: every request arrives at once; continuous batching drains them.
+    sat = None
+    for _ in range(2):
+        st, lat, statuses, stats = server.replay(requests, np.zeros(n, np.uint64))
+        assert st == api.Status.success and (statuses == 0).all(), st
+        sat = stats
+    sat_qps = n / sat.seconds
+    log(f"roi saturation: {sat.seconds * 1e3:.2f} ms, {sat_qps:.0f} requests/s, "
+        f"{sat.decoded_bytes / sat.seconds / 1e9:.1f} GB/s decoded, {sat.batches} batches")
+    rng = np.random.default_rng(args.seed + 6)
+    latency = []
+    for q in args.qps:
+        arrivals = np.cumsum(rng.exponential(1e9 / q, n)).astype(np.uint64)
+        st, lat, statuses, stats = server.replay(requests, arrivals)
+        assert st == api.Status.success and (statuses == 0).all(), st
+        rec = {"qps": q, "achieved_qps": round(n / (arrivals[-1] * 1e-9), 1),
+               "p50_us": round(percentile(lat, 50) / 1e3, 1), "p99_us": round(percentile(lat, 99) / 1e3, 1),
+               "max_us": round(float(lat.max()) / 1e3, 1), "batches": int(stats.batches),
+               "mean_batch": round(n / stats.batches, 1)}
+        log(f"roi latency at {q} req/s: p50 {rec['p50_us']} us, p99 {rec['p99_us']} us, "
+            f"{stats.batches} batches")
+        latency.append(rec)
+
+    # Reference-style path on a bounded sample: one prepare_batch + one
+    # decode_calibrate_async per request, synchronized (closed loop).
+    plain_lat = []
+    ws = api.DeviceMemory(1 << 20)
+    res = api.DeviceMemory(16)
+    sample = min(args.roi_plain_sample, n)
+    for i in range(sample):
+        a, x, y, w, h = rois[i]
+        e = rb.entries[a]
+        it = rb.item(a, (x, y, w, h))
+        it.output = requests[i].output
+        it.calibration.offset = api.scalar(offset)
+        it.calibration.gain = api.scalar(gain)
+        t1 = time.perf_counter()
+        st, plan = api.prepare_batch(api.Operation.decode_calibrate, [it], stream=stream)
+        assert st == api.Status.success, st
+        if plan.workspace_bytes() > ws.size:
+            ws = api.DeviceMemory(plan.workspace_bytes())
+        assert api.decode_calibrate_async(plan, ws, res, 1, stream) == api.Status.success
+        stream.synchronize()
+        plain_lat.append(time.perf_counter() - t1)
+    plain = {"requests": sample, "p50_us": round(percentile(plain_lat, 50) * 1e6, 1),
+             "p99_us": round(percentile(plain_lat, 99) * 1e6, 1),
+             "requests_per_s": round(sample / sum(plain_lat), 1),
+             "note": "one prepare_batch + decode_calibrate_async + stream sync per request, host planning included"}
+    log(f"roi per-request path: p50 {plain['p50_us']} us, p99 {plain['p99_us']} us, "
+        f"{plain['requests_per_s']} requests/s")
+
+    verified = None
+    if not args.no_verify:
+        # Re-run the saturation replay into cleared outputs, then check a sample
+        # against host binary32 (x - 100) * 1.05 of the generated pixels.
+        out.fill(0)
+        st, _, statuses, _ = server.replay(requests, np.zeros(n, np.uint64))
+        ok = st == api.Status.success and bool((statuses == 0).all())
+        big = max(range(n), key=lambda i: rois[i][3] * rois[i][4])
+        o32, g32 = np.float32(offset), np.float32(gain)
+        for i in sorted(set(range(0, n, max(1, n // 200))) | {big, n - 1}):
+            a, x, y, w, h = rois[i]
+            want = ((rb.entries[a].pixels[y:y + h, x:x + w].astype(np.float32) - o32) * g32).view(np.uint32)
+            got = out.download(w * h * 4, offsets[i]).view(np.uint32).reshape(h, w)
+            ok = ok and np.array_equal(got, want)
+        verified = ok
+        log(f"roi verification bit-exact: {ok}")
+    return {
+        "workload": f"{len(rb.entries)} resident {side}x{side} uint16 archives ({tile[0]}x{tile[1]} tiles), "
+                    f"{n} region requests, sides log-uniform in [32, 2048], uniform origins, fused "
+                    f"(x - {offset:g}) * {gain:g} to FP32; Poisson arrivals",
+        "requests": n, "decoded_bytes": decoded, "output_bytes": sum(w * h * 4 for *_, w, h in rois),
+        "saturation": {"ms": round(sat.seconds * 1e3, 3), "requests_per_s": round(sat_qps, 1),
+                       "decoded_GBps": round(sat.decoded_bytes / sat.seconds / 1e9, 2),
+                       "batches": int(sat.batches)},
+        "latency": latency,
+        "per_request_path": plain,
+        "server": {"streams": args.roi_streams, "max_batch": args.roi_max_batch},
+        "verified_bit_exact": verified,
+    }
+
+
 def main() -> int:
     p = argparse.ArgumentParser()
     p.add_argument("--gpus", type=int, default=1)
@@ -99,6 +222,15 @@ def main() -> int:
     p.add_argument("--shape", default="8,256,8,2", help="group,block,chunk,depth used with --profile")
     p.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                    help="only run a plain copy kernel of this many MiB (scales HBM byte counters)")
+    p.add_argument("--no-roi", action="store_true", help="skip the ROI serving leg")
+    p.add_argument("--roi-only", action="store_true", help="only the ROI serving leg")
+    p.add_argument("--roi-archives", type=int, default=64)
+    p.add_argument("--roi-requests", type=int, default=10000)
+    p.add_argument("--roi-streams", type=int, default=3)
+    p.add_argument("--roi-max-batch", type=int, default=1024)
+    p.add_argument("--roi-plain-sample", type=int, default=300)
+    p.add_argument("--qps", type=lambda v: [float(x) for x in v.split(",")], default=[100000.0, 400000.0],
+                   help="Poisson arrival rates (requests/s) for the ROI latency runs")
     args = p.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (this codec has no "
@@ -122,6 +254,10 @@ def main() -> int:
         log(f"copy probe: {args.steps} copies of {nbytes} bytes (read {nbytes}, write {nbytes} each)")
         return 0
     threads = max(1, min(16, os.cpu_count() or 1))
+    if args.roi_only:
+        roi = roi_leg(args, threads, api.Stream())
+        print(json.dumps({"metric": "p99 ROI latency", "roi": roi}), flush=True)
+        return 0 if roi["verified_bit_exact"] in (None, True) else 3
     t0 = time.perf_counter()
     layout = generators.batch_layout(args.images, args.seed)
     batch = ResidentBatch.build(layout, args.seed, threads)
@@ -192,6 +328,8 @@ def main() -> int:
         log(f"verification bit-exact: {ok}")
 
     cpu = None if args.no_cpu else cpu_baseline(batch, threads, 1_000_000_000)
+    del plans, pws, rec
+    roi = None if args.no_roi else roi_leg(args, threads, stream)
     if cpu:
         log(f"cpu baseline {cpu['value']} GB/s on {cpu['cores']} threads ({cpu['kind']})")
 
@@ -233,9 +371,13 @@ def main() -> int:
                      "candidates": len(candidates)},
         "cpu_baseline": cpu,
         "verified_bit_exact": verified,
+        "roi": roi,
     }
+    if roi is not None:
+        out["p99_roi_latency_us"] = {str(int(r["qps"])): r["p99_us"] for r in roi["latency"]}
     print(json.dumps(out), flush=True)
-    return 0 if verified in (None, True) else 3
+    ok = verified in (None, True) and (roi is None or roi["verified_bit_exact"] in (None, True))
+    return 0 if ok else 3
 
 
 if __name__ == "__main__":

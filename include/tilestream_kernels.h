@@ -168,6 +168,10 @@ typedef struct {
 int tsk_results_begin(void *results, uint32_t count, void *stream);
 int tsk_results_complete(const tsk_decode_image *images, uint32_t count, void *results,
                          void *stream);
+/* tsk_results_complete that writes the final records to `out` (for example
+ * mapped host memory) and leaves `results` as the decode left them. */
+int tsk_results_publish(const tsk_decode_image *images, uint32_t count, const void *results, void *out,
+                        void *stream);
 int tsk_decode(const tsk_decode_args *args, void *stream);
 /* Resident workgroups per CU of a decode kernel variant (for persistent grids). */
 int tsk_decode_occupancy(uint32_t kind, uint32_t group, uint32_t block, uint32_t depth, int *blocks);

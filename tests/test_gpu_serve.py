@@ -215,7 +215,7 @@ def test_archive_set_rejects_bad_archives(orc):
     st, _ = s.add(mem, meta, size=len(data) - 4)
     assert st == ts.Status.buffer_too_small
     t = meta.tile(1)
-    t.length += 4
+    t.width -= 1  # no longer the tile's geometry
     meta.set_tile(1, t)
     st, _ = s.add(mem, meta)
     assert st == ts.Status.malformed_data

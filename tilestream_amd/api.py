@@ -491,12 +491,13 @@ class ArchiveSet:
         self.handle = handle.value
         self._keep = []
 
-    def add(self, device_file: DeviceMemory, metadata: Metadata, size: Optional[int] = None):
-        """Adds an archive already in device memory; returns (status, id)."""
+    def add(self, device_file: DeviceMemory, metadata: Metadata, size: Optional[int] = None, offset: int = 0):
+        """Adds an archive already in device memory (at `offset` in the
+        allocation); returns (status, id)."""
         ident = C.c_uint32(0)
-        n = device_file.size if size is None else size
-        st = Status(N.lib().ts_archive_set_add(self.handle, ConstBytes(device_file.ptr, n), metadata.handle,
-                                               C.byref(ident)))
+        n = device_file.size - offset if size is None else size
+        st = Status(N.lib().ts_archive_set_add(self.handle, ConstBytes(device_file.ptr + offset, n),
+                                               metadata.handle, C.byref(ident)))
         if st == Status.success:
             self._keep.append((device_file, metadata))
         return st, ident.value

@@ -41,6 +41,11 @@ def _compile(src: str) -> str:
     path = os.path.join(CSRC, src)
     obj = os.path.join(OBJ, src.replace("/", "_") + ".o")
     flags = COMMON + (HIP_FLAGS if src.endswith(".hip") else HOST_FLAGS)
+    if src.startswith("hip/decode_"):
+        # The work queue's look-ahead draw is one lane's atomic whose result is
+        # awaited later: keep it a plain returning atomic, not a wave-combined
+        # one that waits for its result (and every earlier store) at once.
+        flags = flags + ["-mllvm", "-amdgpu-atomic-optimizer-strategy=None"]
     cmd = [HIPCC] + flags + ["-c", path, "-o", obj]
     res = subprocess.run(cmd, capture_output=True, text=True)
     noise = "is not a recognized feature for this target"

@@ -22,7 +22,8 @@ __global__ void results_begin_kernel(ResultBits *results, std::uint32_t count) {
         results[i] = {0u, 0u, 0u};
 }
 
-__global__ void results_complete_kernel(const tsk_decode_image *images, std::uint32_t count, ResultBits *results) {
+__global__ void results_complete_kernel(const tsk_decode_image *images, std::uint32_t count, const ResultBits *results,
+                                        ResultBits *out) {
     const std::uint64_t stride = std::uint64_t{gridDim.x} * blockDim.x;
     for (std::uint64_t i = std::uint64_t{blockIdx.x} * blockDim.x + threadIdx.x; i < count; i += stride) {
         const tsk_decode_image &img = images[i];
@@ -32,7 +33,7 @@ __global__ void results_complete_kernel(const tsk_decode_image *images, std::uin
             if (produced != 0) status = produced;
         }
         const std::uint64_t s = img.calibrated ? 4 : (img.type == TSK_UINT16 ? 2 : 4);
-        results[i] = {status, 0u, status == 0 ? std::uint64_t{img.roi_w} * img.roi_h * s : 0u};
+        out[i] = {status, 0u, status == 0 ? std::uint64_t{img.roi_w} * img.roi_h * s : 0u};
     }
 }
 
@@ -80,7 +81,17 @@ int tsk_results_begin(void *results, std::uint32_t count, void *stream) {
 int tsk_results_complete(const tsk_decode_image *images, std::uint32_t count, void *results, void *stream) {
     if (count == 0) return hipSuccess;
     hipLaunchKernelGGL(results_complete_kernel, dim3(small_grid(count)), dim3(256), 0,
-                       static_cast<hipStream_t>(stream), images, count, static_cast<ResultBits *>(results));
+                       static_cast<hipStream_t>(stream), images, count, static_cast<const ResultBits *>(results),
+                       static_cast<ResultBits *>(results));
+    return hipGetLastError();
+}
+
+int tsk_results_publish(const tsk_decode_image *images, std::uint32_t count, const void *results, void *out,
+                        void *stream) {
+    if (count == 0) return hipSuccess;
+    hipLaunchKernelGGL(results_complete_kernel, dim3(small_grid(count)), dim3(256), 0,
+                       static_cast<hipStream_t>(stream), images, count, static_cast<const ResultBits *>(results),
+                       static_cast<ResultBits *>(out));
     return hipGetLastError();
 }
 
@@ -98,7 +109,9 @@ int tsk_decode(const tsk_decode_args *args, void *stream) {
 
 int tsk_copy_blocks(void *dst, const void *src, std::uint64_t blocks, void *stream) {
     if (blocks == 0) return hipSuccess;
-    hipLaunchKernelGGL(copy_kernel, dim3(4096), dim3(256), 0, static_cast<hipStream_t>(stream),
+    const std::uint64_t groups = (blocks + 255) / 256;
+    hipLaunchKernelGGL(copy_kernel, dim3(groups < 4096 ? static_cast<unsigned>(groups) : 4096u), dim3(256), 0,
+                       static_cast<hipStream_t>(stream),
                        static_cast<uint4 *>(dst), static_cast<const uint4 *>(src), blocks);
     return hipGetLastError();
 }

@@ -20,8 +20,9 @@
 //    carry the running value across passes.
 //  * uint16 reconstruction is a 32-bit add of inverse-zigzag terms truncated to
 //    16 bits at the store (exact modulo 2^16); float32 is a running XOR of bit
-//    words.  Groups fully inside the region are written with 16-byte stores
-//    when aligned.
+//    words.  Groups fully inside the region are written with 8- and 16-byte
+//    stores at any word boundary; 16-bit outputs at odd halfword boundaries and
+//    groups on the region's edge store sample by sample.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -55,6 +56,21 @@ template <typename T> using gptr = T __attribute__((address_space(1))) *;
 typedef std::uint32_t u32x4 __attribute__((ext_vector_type(4)));
 typedef std::uint32_t u32x2 __attribute__((ext_vector_type(2)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
+// Multi-word global accesses need only word alignment on CDNA: these types let
+// output runs at any 4-byte boundary use 8- and 16-byte stores.
+typedef std::uint32_t u32x4w __attribute__((ext_vector_type(4), aligned(4)));
+typedef std::uint32_t u32x2w __attribute__((ext_vector_type(2), aligned(4)));
+
+// LDS words addressed by a computed byte address.
+using lds_word = const __attribute__((address_space(3))) std::uint32_t;
+
+__device__ inline std::uint32_t lds_address(const std::uint32_t *p) {
+    return static_cast<std::uint32_t>(reinterpret_cast<std::uintptr_t>((lds_word *)p));
+}
+
+__device__ inline lds_word *lds_at(std::uint32_t byte_address) {
+    return reinterpret_cast<lds_word *>(static_cast<std::uintptr_t>(byte_address));
+}
 
 template <typename T> __device__ inline gptr<T> global(T *p) {
     return reinterpret_cast<gptr<T>>(reinterpret_cast<std::uintptr_t>(p));
@@ -143,51 +159,98 @@ __device__ inline float sample_float(std::uint32_t bits, bool u16) {
     return u16 ? static_cast<float>(bits & 0xffffu) : __uint_as_float(bits);
 }
 
+// Stores G consecutive 32-bit words at a 4-byte aligned address.  Outputs are
+// written once and never read back by the decoder, so a group of at most 16
+// bytes is stored non-temporally: consecutive lanes then cover whole cache
+// lines in one instruction.  Wider groups take several instructions per line
+// and keep ordinary stores, which the L2 merges into whole lines.
+template <int G> __device__ inline void store_words(gptr<std::uint32_t> dst, const std::uint32_t (&w)[G]) {
+    if constexpr (G > 4) {
+#pragma unroll
+        for (int k = 0; k + 4 <= G; k += 4)
+            *reinterpret_cast<gptr<u32x4w>>(dst + k) = u32x4w{w[k], w[k + 1], w[k + 2], w[k + 3]};
+    } else if constexpr (G == 4) {
+        __builtin_nontemporal_store(u32x4w{w[0], w[1], w[2], w[3]}, reinterpret_cast<gptr<u32x4w>>(dst));
+    } else if constexpr (G == 2) {
+        __builtin_nontemporal_store(u32x2w{w[0], w[1]}, reinterpret_cast<gptr<u32x2w>>(dst));
+    } else {
+        __builtin_nontemporal_store(w[0], dst);
+    }
+}
+
+// Loads G consecutive FP32 map values at a 4-byte aligned address.
+template <int G> __device__ inline void load_floats(gptr<const float> src, float (&v)[G]) {
+    if constexpr (G >= 4) {
+#pragma unroll
+        for (int k = 0; k + 4 <= G; k += 4) {
+            const u32x4w q = *reinterpret_cast<gptr<const u32x4w>>(src + k);
+            v[k] = __uint_as_float(q.x);
+            v[k + 1] = __uint_as_float(q.y);
+            v[k + 2] = __uint_as_float(q.z);
+            v[k + 3] = __uint_as_float(q.w);
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < G; ++k) v[k] = src[k];
+    }
+}
+
 // Stores G samples (or their calibrated values) for tile columns col0..col0+G-1.
+// Groups wholly inside the region store whole runs; groups on its left or
+// right edge store only the region's columns.
 template <int G, bool U16, bool CAL>
 __device__ inline void emit(const tsk_decode_image &img, const Region &r, std::uint32_t col0, const Group<G> &g) {
     const bool inside = col0 >= r.left && col0 + G <= r.right;
     const std::int64_t ex0 = r.ex_tile + col0;
     if constexpr (CAL) {
-        float out[G];
-        const gptr<const std::uint8_t> orow = img.offset_map ? global(img.offset_map) + r.ey * img.offset_pitch : nullptr;
-        const gptr<const std::uint8_t> grow = img.gain_map ? global(img.gain_map) + r.ey * img.gain_pitch : nullptr;
-#pragma unroll
-        for (int k = 0; k < G; ++k) {
-            const std::uint32_t c = col0 + k;
-            float o = img.offset, gn = img.gain;
-            if (c >= r.left && c < r.right) {
-                if (orow) o = reinterpret_cast<gptr<const float>>(orow)[ex0 + k];
-                if (grow) gn = reinterpret_cast<gptr<const float>>(grow)[ex0 + k];
-            }
-            // Two roundings to nearest even, never contracted into an FMA.
-            out[k] = __fmul_rn(__fsub_rn(sample_float(g.v[k], U16), o), gn);
-        }
         gptr<float> dst = reinterpret_cast<gptr<float>>(r.line) + ex0;
-        if (inside && G >= 4 && (reinterpret_cast<std::uintptr_t>(dst) & 15u) == 0) {
+        const bool omap = img.offset_map != nullptr, gmap = img.gain_map != nullptr;
+        const gptr<const float> orow =
+            omap ? reinterpret_cast<gptr<const float>>(global(img.offset_map) + r.ey * img.offset_pitch) + ex0 : nullptr;
+        const gptr<const float> grow =
+            gmap ? reinterpret_cast<gptr<const float>>(global(img.gain_map) + r.ey * img.gain_pitch) + ex0 : nullptr;
+        if (inside) {
+            float o[G], gn[G];
+            if (omap) {
+                load_floats<G>(orow, o);
+            } else {
 #pragma unroll
-            for (int k = 0; k + 4 <= G; k += 4)
-                *reinterpret_cast<gptr<f32x4>>(dst + k) = f32x4{out[k], out[k + 1], out[k + 2], out[k + 3]};
-        } else {
+                for (int k = 0; k < G; ++k) o[k] = img.offset;
+            }
+            if (gmap) {
+                load_floats<G>(grow, gn);
+            } else {
+#pragma unroll
+                for (int k = 0; k < G; ++k) gn[k] = img.gain;
+            }
+            std::uint32_t out[G];
 #pragma unroll
             for (int k = 0; k < G; ++k)
-                if (col0 + k >= r.left && col0 + k < r.right) dst[k] = out[k];
+                // Two roundings to nearest even, never contracted into an FMA.
+                out[k] = __float_as_uint(__fmul_rn(__fsub_rn(sample_float(g.v[k], U16), o[k]), gn[k]));
+            store_words<G>(reinterpret_cast<gptr<std::uint32_t>>(dst), out);
+        } else {
+#pragma unroll
+            for (int k = 0; k < G; ++k) {
+                const std::uint32_t c = col0 + k;
+                if (c >= r.left && c < r.right) {
+                    const float o = omap ? orow[k] : img.offset;
+                    const float gn = gmap ? grow[k] : img.gain;
+                    dst[k] = __fmul_rn(__fsub_rn(sample_float(g.v[k], U16), o), gn);
+                }
+            }
         }
     } else if constexpr (U16) {
         gptr<std::uint16_t> dst = reinterpret_cast<gptr<std::uint16_t>>(r.line) + ex0;
-        constexpr int kBytes = 2 * G;
-        if (G >= 2 && inside && (reinterpret_cast<std::uintptr_t>(dst) & ((kBytes >= 16 ? 16 : kBytes) - 1)) == 0) {
-            std::uint32_t w[G / 2 > 0 ? G / 2 : 1];
+        if (inside) {
+            if (G >= 2 && (reinterpret_cast<std::uintptr_t>(dst) & 3u) == 0) {
+                std::uint32_t w[G / 2 > 0 ? G / 2 : 1];
 #pragma unroll
-            for (int i = 0; i < G / 2; ++i) w[i] = (g.v[2 * i] & 0xffffu) | (g.v[2 * i + 1] << 16);
-            if constexpr (G >= 8) {
+                for (int i = 0; i < G / 2; ++i) w[i] = __builtin_amdgcn_perm(g.v[2 * i + 1], g.v[2 * i], 0x05040100u);
+                if constexpr (G >= 2) store_words<G / 2>(reinterpret_cast<gptr<std::uint32_t>>(dst), w);
+            } else {
 #pragma unroll
-                for (int i = 0; i + 4 <= G / 2; i += 4)
-                    *reinterpret_cast<gptr<u32x4>>(dst + 2 * i) = u32x4{w[i], w[i + 1], w[i + 2], w[i + 3]};
-            } else if constexpr (G == 4) {
-                *reinterpret_cast<gptr<u32x2>>(dst) = u32x2{w[0], w[1]};
-            } else if constexpr (G == 2) {
-                *reinterpret_cast<gptr<std::uint32_t>>(dst) = w[0];
+                for (int k = 0; k < G; ++k) dst[k] = static_cast<std::uint16_t>(g.v[k]);
             }
         } else {
 #pragma unroll
@@ -196,17 +259,8 @@ __device__ inline void emit(const tsk_decode_image &img, const Region &r, std::u
         }
     } else {
         gptr<std::uint32_t> dst = reinterpret_cast<gptr<std::uint32_t>>(r.line) + ex0;
-        constexpr int kBytes = 4 * G;
-        if (inside && (reinterpret_cast<std::uintptr_t>(dst) & ((kBytes >= 16 ? 16 : kBytes) - 1)) == 0) {
-            if constexpr (G >= 4) {
-#pragma unroll
-                for (int k = 0; k + 4 <= G; k += 4)
-                    *reinterpret_cast<gptr<u32x4>>(dst + k) = u32x4{g.v[k], g.v[k + 1], g.v[k + 2], g.v[k + 3]};
-            } else if constexpr (G == 2) {
-                *reinterpret_cast<gptr<u32x2>>(dst) = u32x2{g.v[0], g.v[1]};
-            } else {
-                dst[0] = g.v[0];
-            }
+        if (inside) {
+            store_words<G>(dst, g.v);
         } else {
 #pragma unroll
             for (int k = 0; k < G; ++k)
@@ -224,6 +278,92 @@ struct TileView {
 };
 
 // Decodes the staged tile and writes its region pixels.
+// Per-row constants of a predictive row.
+struct RowBits {
+    std::uint32_t seed;
+    std::uint32_t b;
+    std::uint32_t mask;      // low b bits
+    std::uint32_t high, low; // uint16 extract widths: b - 1 and 1 (both 0 for b = 0)
+    std::uint32_t words_bit; // LDS bit address of the row's first packed word
+};
+
+__device__ inline RowBits row_bits(const std::uint32_t *payload, std::uint32_t start) {
+    RowBits r;
+    r.seed = payload[start];
+    r.b = payload[start + 1];
+    r.mask = r.b >= 32 ? 0xffffffffu : ((1u << r.b) - 1u);
+    // uint16 residuals are split by two bit-field extracts: bits 1..b-1
+    // (z >> 1) and bit 0 sign-extended (-(z & 1)); both widths are 0 for
+    // b = 0, which extracts 0.
+    r.high = r.b > 0 ? r.b - 1 : 0u;
+    r.low = r.b > 0 ? 1u : 0u;
+    // A multiple of 32, so the low 5 bits of a bit address are the in-word offset.
+    r.words_bit = lds_address(payload + start + 2) << 3;
+    return r;
+}
+
+// Reconstructs columns col0..col0+G-1 of a predictive row: the lane's G
+// residuals are summed sequentially, the lane totals are scanned over the
+// segment of S lanes, and `carry` is the value before column 0 of this pass.
+// Returns the segment's inclusive total in `incl`.
+template <int G, bool U16>
+__device__ inline void predict_group(const RowBits &rb, std::uint32_t col0, std::uint32_t p, std::uint32_t S,
+                                     std::uint32_t carry, Group<G> &v, std::uint32_t &incl) {
+    // Residual j of this lane is residual col0 + j of the row: it moves from
+    // column col0 + j to col0 + j + 1.  Bit addresses are below 2^21, so 24-bit
+    // multiplies (full rate) suffice.
+    const std::uint32_t bit0 = __umul24(col0, rb.b) + rb.words_bit;
+    std::uint32_t acc = 0;
+    Group<G> pre;
+#pragma unroll
+    for (int j = 0; j < G; ++j) {
+        const std::uint32_t bit = __umul24(static_cast<std::uint32_t>(j), rb.b) + bit0;
+        lds_word *w = lds_at((bit >> 3) & ~3u);
+        const std::uint32_t x = __builtin_amdgcn_alignbit(w[1], w[0], bit);
+        if constexpr (U16) {
+            const std::uint32_t half = __builtin_amdgcn_ubfe(x, 1u, rb.high);
+            const std::uint32_t sign = static_cast<std::uint32_t>(__builtin_amdgcn_sbfe(static_cast<int>(x), 0u, rb.low));
+            acc += half ^ sign;
+        } else {
+            acc ^= x & rb.mask;
+        }
+        pre.v[j] = acc;
+    }
+    incl = segment_scan<U16>(acc, p, S);
+    const std::uint32_t first = Ring<U16>::add(carry, Ring<U16>::sub(incl, acc));
+    v.v[0] = first;
+#pragma unroll
+    for (int k = 1; k < G; ++k) v.v[k] = Ring<U16>::add(first, pre.v[k - 1]);
+}
+
+// Columns col0..col0+G-1 of a RAW (mode 0) or CONSTANT (mode 1) tile row.
+template <int G, bool U16>
+__device__ inline void stored_group(const TileView &t, const std::uint32_t *payload, std::uint32_t row,
+                                    std::uint32_t col0, Group<G> &v) {
+    if (t.mode == 0) {
+        const std::uint32_t idx0 = __umul24(row, t.w) + col0;
+#pragma unroll
+        for (int k = 0; k < G; ++k) {
+            const std::uint32_t idx = idx0 + k;
+            v.v[k] = U16 ? (payload[idx >> 1] >> ((idx & 1u) * 16)) : payload[idx];
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < G; ++k) v.v[k] = payload[0];
+    }
+}
+
+__device__ inline Region region_of(const tsk_decode_image &img, const TileView &t, std::uint32_t row) {
+    Region r;
+    r.ey = t.ty + row - img.roi_y;
+    r.line = global(img.out) + r.ey * img.out_pitch;
+    r.ex_tile = static_cast<std::int64_t>(t.tx) - static_cast<std::int64_t>(img.roi_x);
+    r.left = t.left;
+    r.right = t.right;
+    return r;
+}
+
+// Decodes the staged tile and writes its region pixels as it goes.
 template <int G, bool U16, bool CAL, int BLOCK>
 __device__ void decode_tile(const tsk_decode_image &img, const TileView &t, const std::uint32_t *stage,
                             const std::uint16_t *row_start) {
@@ -234,65 +374,76 @@ __device__ void decode_tile(const tsk_decode_image &img, const TileView &t, cons
     const std::uint32_t *payload = stage + t.base;
 
     for (std::uint32_t row = t.top + threadIdx.x / S; row < t.bottom; row += rows_per_step) {
-        Region r;
-        r.ey = t.ty + row - img.roi_y;
-        r.line = global(img.out) + r.ey * img.out_pitch;
-        r.ex_tile = static_cast<std::int64_t>(t.tx) - static_cast<std::int64_t>(img.roi_x);
-        r.left = t.left;
-        r.right = t.right;
+        const Region r = region_of(img, t, row);
         if (t.mode == 2) {
-            const std::uint32_t start = row_start[row];
-            const std::uint32_t seed = payload[start];
-            const std::uint32_t b = payload[start + 1];
-            const std::uint32_t mask = b >= 32 ? 0xffffffffu : ((1u << b) - 1u);
-            const std::uint32_t *words = payload + start + 2;
-            std::uint32_t carry = seed;
+            const RowBits rb = row_bits(payload, row_start[row]);
+            std::uint32_t carry = rb.seed;
             for (std::uint32_t pass = 0; pass < passes; ++pass) {
                 const std::uint32_t col0 = (pass * S + p) * G;
-                // Residual j of this lane is residual col0 + j of the row: it moves
-                // from column col0 + j to col0 + j + 1.  Bit offsets are below
-                // 2^13, so 24-bit multiplies (full rate) suffice.
-                const std::uint32_t bit0 = __umul24(col0, b);
-                std::uint32_t acc = 0;
-                Group<G> pre;
-#pragma unroll
-                for (int j = 0; j < G; ++j) {
-                    const std::uint32_t bit = __umul24(static_cast<std::uint32_t>(j), b) + bit0;
-                    const std::uint32_t w = bit >> 5;
-                    const std::uint32_t z = __builtin_amdgcn_alignbit(words[w + 1], words[w], bit) & mask;
-                    acc = Ring<U16>::add(acc, Ring<U16>::term(z));
-                    pre.v[j] = acc;
-                }
-                const std::uint32_t incl = segment_scan<U16>(acc, p, S);
-                const std::uint32_t first = Ring<U16>::add(carry, Ring<U16>::sub(incl, acc));
-                if (passes > 1)
+                Group<G> v;
+                std::uint32_t incl;
+                predict_group<G, U16>(rb, col0, p, S, carry, v, incl);
+                if (passes > 1) // S = 64: lane 63 holds the row total so far
                     carry = Ring<U16>::add(carry, static_cast<std::uint32_t>(
                                                       __builtin_amdgcn_readlane(static_cast<int>(incl), 63)));
-                Group<G> v;
-                v.v[0] = first;
-#pragma unroll
-                for (int k = 1; k < G; ++k) v.v[k] = Ring<U16>::add(first, pre.v[k - 1]);
                 emit<G, U16, CAL>(img, r, col0, v);
             }
         } else {
             for (std::uint32_t pass = 0; pass < passes; ++pass) {
                 const std::uint32_t col0 = (pass * S + p) * G;
                 Group<G> v;
-                if (t.mode == 0) {
-                    const std::uint32_t idx0 = __umul24(row, t.w) + col0;
-#pragma unroll
-                    for (int k = 0; k < G; ++k) {
-                        const std::uint32_t idx = idx0 + k;
-                        v.v[k] = U16 ? (payload[idx >> 1] >> ((idx & 1u) * 16)) : payload[idx];
-                    }
-                } else {
-#pragma unroll
-                    for (int k = 0; k < G; ++k) v.v[k] = payload[0];
-                }
+                stored_group<G, U16>(t, payload, row, col0, v);
                 emit<G, U16, CAL>(img, r, col0, v);
             }
         }
     }
+}
+
+// Groups per thread of a 4096-pixel tile when G >= 4 (one pass per row).
+template <int G, int BLOCK> constexpr int held_groups() {
+    return (4096 + BLOCK * G - 1) / (BLOCK * G);
+}
+
+// A thread's decoded groups of one tile, kept in registers until they are
+// stored (row >= the tile's bottom: no group).
+template <int G, int K> struct Held {
+    Group<G> v[K];
+    std::uint32_t row[K];
+};
+
+// Decodes the staged tile into registers (G >= 4, so S <= 64 and one pass per
+// row); held_store writes them afterwards.
+template <int G, bool U16, int BLOCK, int K>
+__device__ inline void decode_tile_held(const tsk_decode_image &img, const TileView &t, const std::uint32_t *stage,
+                                        const std::uint16_t *row_start, Held<G, K> &h) {
+    static_assert(G >= 4, "one pass per row");
+    const std::uint32_t S = img.tile_w / G;
+    const std::uint32_t p = threadIdx.x % S;
+    const std::uint32_t rows_per_step = BLOCK / S;
+    const std::uint32_t col0 = p * G;
+    const std::uint32_t *payload = stage + t.base;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        const std::uint32_t row = t.top + threadIdx.x / S + k * rows_per_step;
+        h.row[k] = row;
+        if (row < t.bottom) {
+            if (t.mode == 2) {
+                const RowBits rb = row_bits(payload, row_start[row]);
+                std::uint32_t incl;
+                predict_group<G, U16>(rb, col0, p, S, rb.seed, h.v[k], incl);
+            } else {
+                stored_group<G, U16>(t, payload, row, col0, h.v[k]);
+            }
+        }
+    }
+}
+
+template <int G, bool U16, bool CAL, int BLOCK, int K>
+__device__ inline void held_store(const tsk_decode_image &img, const TileView &t, const Held<G, K> &h) {
+    const std::uint32_t col0 = (threadIdx.x % (img.tile_w / G)) * G;
+#pragma unroll
+    for (int k = 0; k < K; ++k)
+        if (h.row[k] < t.bottom) emit<G, U16, CAL>(img, region_of(img, t, h.row[k]), col0, h.v[k]);
 }
 
 // A tile whose payload is being fetched.
@@ -419,13 +570,91 @@ __device__ inline std::uint64_t draw(const tsk_decode_args &a, Cursor &c, std::u
     return c.next++;
 }
 
-template <int G, int KIND, int BLOCK, int DEPTH>
-__global__ void __launch_bounds__(BLOCK) decode_kernel(tsk_decode_args a) {
+// Validates the row descriptors of a trusted (encoder-produced) predictive
+// tile staged in LDS and fills row_start: a serial walk in thread 0
+// (reference decoder.cu:86-110).  Returns whether the tile is well formed.
+__device__ inline bool walk_rows(const Pending &c, bool u16, const std::uint32_t *stage, std::uint16_t *row_start,
+                                 std::uint32_t *shared_flag) {
+    if (threadIdx.x == 0) {
+        const std::uint32_t widest = u16 ? 16u : 32u;
+        std::uint32_t cur = 0;
+        bool ok = true;
+        for (std::uint32_t r = 0; r < c.h && ok; ++r) {
+            if (c.words - cur < 2) {
+                ok = false;
+                break;
+            }
+            const std::uint32_t seed = stage[c.mis + cur];
+            const std::uint32_t bits = stage[c.mis + cur + 1];
+            const std::uint32_t packed = ((c.w - 1) * bits + 31) / 32;
+            if (bits > widest || (u16 && seed > 65535u) || c.words - cur - 2 < packed) {
+                ok = false;
+                break;
+            }
+            row_start[r] = static_cast<std::uint16_t>(cur);
+            cur += 2 + packed;
+        }
+        *shared_flag = ok && cur == c.words;
+    }
+    __syncthreads();
+    return *shared_flag != 0;
+}
+
+__device__ inline TileView tile_view(const Pending &c, const tsk_decode_image &img) {
+    TileView t;
+    t.tx = std::uint64_t{c.column} * img.tile_w;
+    t.ty = std::uint64_t{c.row} * img.tile_h;
+    t.w = c.w;
+    t.h = c.h;
+    t.mode = c.mode;
+    t.base = c.mis;
+    const std::uint64_t rx = img.roi_x, ry = img.roi_y;
+    t.top = ry > t.ty ? static_cast<std::uint32_t>(ry - t.ty) : 0u;
+    t.bottom = ry + img.roi_h < t.ty + t.h ? static_cast<std::uint32_t>(ry + img.roi_h - t.ty) : t.h;
+    t.left = rx > t.tx ? static_cast<std::uint32_t>(rx - t.tx) : 0u;
+    t.right = rx + img.roi_w < t.tx + t.w ? static_cast<std::uint32_t>(rx + img.roi_w - t.tx) : t.w;
+    return t;
+}
+
+// Makes the prefetched payload registers and the look-ahead queue draw
+// available: the wait lands here, before this tile's output stores are
+// issued, so it never waits for them (vector memory counters retire in order).
+template <int KMAX>
+__device__ inline void await_prefetch(const u32x4 (&reg)[KMAX], std::uint32_t row_reg, std::uint64_t ahead) {
+#pragma unroll
+    for (int k = 0; k < KMAX; ++k) asm volatile("" ::"v"(reg[k]) : "memory");
+    asm volatile("" ::"v"(row_reg), "v"(static_cast<std::uint32_t>(ahead)), "v"(static_cast<std::uint32_t>(ahead >> 32))
+                 : "memory");
+}
+
+// Loads a tile's staged blocks and row offsets into registers.  Every lane
+// loads every slot, clamped to the tile's last block (or to the first block
+// of the task table for a tile without blocks), so that no load is
+// conditional: the register array then stays in place instead of being
+// copied at each branch.
+template <int BLOCK, int KMAX>
+__device__ inline void prefetch(const tsk_decode_args &a, const Pending &p, u32x4 (&reg)[KMAX], std::uint32_t &row_reg) {
+    const gptr<const u32x4> base =
+        p.nblocks != 0 ? p.blocks : reinterpret_cast<gptr<const u32x4>>(reinterpret_cast<std::uintptr_t>(a.tasks));
+    const std::uint32_t last = p.nblocks != 0 ? p.nblocks - 1 : 0u;
+#pragma unroll
+    for (int k = 0; k < KMAX; ++k) {
+        const std::uint32_t b = threadIdx.x + k * BLOCK;
+        reg[k] = __builtin_nontemporal_load(base + (b < last ? b : last));
+    }
+    const bool rows = p.mode == 2 && p.rows != TSK_NO_ROWS && p.rows < a.row_count;
+    const std::uint64_t r = rows ? p.rows + (threadIdx.x < p.h ? threadIdx.x : 0u) : 0u;
+    const std::uint64_t r_clamped = r < a.row_count ? r : p.rows;
+    row_reg = rows ? (global(a.rows) + r_clamped)[0] : 0u;
+}
+
+// Depth 1: the payload of a tile is loaded after the previous one is decoded.
+template <int G, int KIND, int BLOCK>
+__device__ inline __attribute__((always_inline)) void decode_body_staged(const tsk_decode_args &a) {
     __shared__ __attribute__((aligned(16))) std::uint32_t stage[kStageWords];
     __shared__ std::uint16_t row_start[64];
     __shared__ std::uint32_t shared_flag;
     __shared__ std::uint64_t shared_next[1];
-    constexpr int KMAX = DEPTH > 1 ? static_cast<int>((kMaxBlocks + BLOCK - 1) / BLOCK) : 1;
 
     const std::uint32_t tid = threadIdx.x;
     Cursor cursor{blockIdx.x, 0, 0};
@@ -433,35 +662,81 @@ __global__ void __launch_bounds__(BLOCK) decode_kernel(tsk_decode_args a) {
         if (tid == 0) cursor.ahead = atomicAdd(reinterpret_cast<unsigned long long *>(a.queue), a.chunk);
         cursor.next = cursor.end = 0;
     }
-    std::uint64_t i = draw<BLOCK>(a, cursor, shared_next);
+    for (;;) {
+        const std::uint64_t i = draw<BLOCK>(a, cursor, shared_next);
+        if (i >= a.task_count) return;
+        const Pending c = describe(a, i);
+        __syncthreads(); // the previous tile is done with the LDS
+        for (std::uint32_t k = tid; k < c.nblocks; k += BLOCK) *reinterpret_cast<u32x4 *>(stage + 4 * k) = c.blocks[k];
+        if (c.mode == 2 && c.rows != TSK_NO_ROWS && tid < c.h && c.rows + tid < a.row_count)
+            row_start[tid] = (global(a.rows) + c.rows)[tid];
+        __syncthreads();
+        if (c.state == 2) {
+            if (tid == 0) fail(a.results, c.image, kMalformed);
+            continue;
+        }
+        if (c.state != 0) continue;
+        // The descriptor is copied by value: output stores cannot alias it.
+        const tsk_decode_image img = load_constant(a.images, c.image);
+        const bool u16 = img.type == TSK_UINT16;
+        if (c.mode == 2 && c.rows == TSK_NO_ROWS && !walk_rows(c, u16, stage, row_start, &shared_flag)) {
+            if (tid == 0) fail(a.results, c.image, kMalformed);
+            continue;
+        }
+        const TileView t = tile_view(c, img);
+        if constexpr (KIND == TSK_KIND_U16) {
+            decode_tile<G, true, false, BLOCK>(img, t, stage, row_start);
+        } else if constexpr (KIND == TSK_KIND_F32) {
+            decode_tile<G, false, false, BLOCK>(img, t, stage, row_start);
+        } else if constexpr (KIND == TSK_KIND_U16_CAL) {
+            decode_tile<G, true, true, BLOCK>(img, t, stage, row_start);
+        } else if constexpr (KIND == TSK_KIND_F32_CAL) {
+            decode_tile<G, false, true, BLOCK>(img, t, stage, row_start);
+        } else if (u16) {
+            if (img.calibrated) decode_tile<G, true, true, BLOCK>(img, t, stage, row_start);
+            else decode_tile<G, true, false, BLOCK>(img, t, stage, row_start);
+        } else {
+            if (img.calibrated) decode_tile<G, false, true, BLOCK>(img, t, stage, row_start);
+            else decode_tile<G, false, false, BLOCK>(img, t, stage, row_start);
+        }
+    }
+}
+
+// Depth 2: while a tile is decoded into registers the next tile's payload is
+// loaded into registers; after the wait for it, the decoded groups are stored
+// and the payload is committed to LDS for the next round.
+template <int G, int KIND, int BLOCK>
+__device__ inline __attribute__((always_inline)) void decode_body_prefetch(const tsk_decode_args &a) {
+    __shared__ __attribute__((aligned(16))) std::uint32_t stage[kStageWords];
+    __shared__ std::uint16_t row_start[64];
+    __shared__ std::uint32_t shared_flag;
+    __shared__ std::uint64_t shared_next[1];
+    constexpr int KMAX = static_cast<int>((kMaxBlocks + BLOCK - 1) / BLOCK);
+    constexpr int K = held_groups<G, BLOCK>();
+
+    const std::uint32_t tid = threadIdx.x;
+    Cursor cursor{blockIdx.x, 0, 0};
+    if (a.queue != nullptr) {
+        if (tid == 0) cursor.ahead = atomicAdd(reinterpret_cast<unsigned long long *>(a.queue), a.chunk);
+        cursor.next = cursor.end = 0;
+    }
+    const std::uint64_t i = draw<BLOCK>(a, cursor, shared_next);
     if (i >= a.task_count) return;
 
     Pending p = describe(a, i);
     u32x4 reg[KMAX];
     std::uint32_t row_reg = 0;
-    if constexpr (DEPTH > 1) {
-#pragma unroll
-        for (int k = 0; k < KMAX; ++k)
-            if (tid + k * BLOCK < p.nblocks) reg[k] = p.blocks[tid + k * BLOCK];
-        if (p.mode == 2 && p.rows != TSK_NO_ROWS && tid < p.h && p.rows + tid < a.row_count)
-            row_reg = (global(a.rows) + p.rows)[tid];
-    }
+    prefetch<BLOCK>(a, p, reg, row_reg);
+    await_prefetch(reg, row_reg, cursor.ahead);
 
     for (;;) {
         const Pending c = p;
         __syncthreads(); // the previous tile is done with the LDS
-        if constexpr (DEPTH > 1) {
 #pragma unroll
-            for (int k = 0; k < KMAX; ++k)
-                if (tid + k * BLOCK < c.nblocks) *reinterpret_cast<u32x4 *>(stage + 4 * (tid + k * BLOCK)) = reg[k];
-            if (c.mode == 2 && c.rows != TSK_NO_ROWS && tid < c.h && c.rows + tid < a.row_count)
-                row_start[tid] = static_cast<std::uint16_t>(row_reg);
-        } else {
-            for (std::uint32_t k = tid; k < c.nblocks; k += BLOCK)
-                *reinterpret_cast<u32x4 *>(stage + 4 * k) = c.blocks[k];
-            if (c.mode == 2 && c.rows != TSK_NO_ROWS && tid < c.h && c.rows + tid < a.row_count)
-                row_start[tid] = (global(a.rows) + c.rows)[tid];
-        }
+        for (int k = 0; k < KMAX; ++k)
+            if (tid + k * BLOCK < c.nblocks) *reinterpret_cast<u32x4 *>(stage + 4 * (tid + k * BLOCK)) = reg[k];
+        if (c.mode == 2 && c.rows != TSK_NO_ROWS && tid < c.h && c.rows + tid < a.row_count)
+            row_start[tid] = static_cast<std::uint16_t>(row_reg);
         __syncthreads();
 
         // Start fetching the next tile; its loads are in flight while this one decodes.
@@ -469,84 +744,66 @@ __global__ void __launch_bounds__(BLOCK) decode_kernel(tsk_decode_args a) {
         const bool more = n < a.task_count;
         if (more) {
             p = describe(a, n);
-            if constexpr (DEPTH > 1) {
-#pragma unroll
-                for (int k = 0; k < KMAX; ++k)
-                    if (tid + k * BLOCK < p.nblocks) reg[k] = p.blocks[tid + k * BLOCK];
-                if (p.mode == 2 && p.rows != TSK_NO_ROWS && tid < p.h && p.rows + tid < a.row_count)
-                    row_reg = (global(a.rows) + p.rows)[tid];
-            }
+            prefetch<BLOCK>(a, p, reg, row_reg);
         }
 
+        bool store = false;
+        tsk_decode_image img{};
+        TileView t{};
+        Held<G, K> h;
         if (c.state == 2) {
             if (tid == 0) fail(a.results, c.image, kMalformed);
         } else if (c.state == 0) {
             // The descriptor is copied by value: output stores cannot alias it.
-            const tsk_decode_image img = load_constant(a.images, c.image);
+            img = load_constant(a.images, c.image);
             const bool u16 = img.type == TSK_UINT16;
-            bool good = true;
-            if (c.mode == 2 && c.rows == TSK_NO_ROWS) {
-                // Trusted encoder output: serial walk of the row descriptors in LDS
-                // (reference decoder.cu:86-110).
-                if (tid == 0) {
-                    const std::uint32_t widest = u16 ? 16u : 32u;
-                    std::uint32_t cur = 0;
-                    bool ok = true;
-                    for (std::uint32_t r = 0; r < c.h && ok; ++r) {
-                        if (c.words - cur < 2) {
-                            ok = false;
-                            break;
-                        }
-                        const std::uint32_t seed = stage[c.mis + cur];
-                        const std::uint32_t bits = stage[c.mis + cur + 1];
-                        const std::uint32_t packed = ((c.w - 1) * bits + 31) / 32;
-                        if (bits > widest || (u16 && seed > 65535u) || c.words - cur - 2 < packed) {
-                            ok = false;
-                            break;
-                        }
-                        row_start[r] = static_cast<std::uint16_t>(cur);
-                        cur += 2 + packed;
-                    }
-                    shared_flag = ok && cur == c.words;
-                }
-                __syncthreads();
-                good = shared_flag != 0;
-                if (!good && tid == 0) fail(a.results, c.image, kMalformed);
-            }
-            if (good) {
-                TileView t;
-                t.tx = std::uint64_t{c.column} * img.tile_w;
-                t.ty = std::uint64_t{c.row} * img.tile_h;
-                t.w = c.w;
-                t.h = c.h;
-                t.mode = c.mode;
-                t.base = c.mis;
-                const std::uint64_t rx = img.roi_x, ry = img.roi_y;
-                t.top = ry > t.ty ? static_cast<std::uint32_t>(ry - t.ty) : 0u;
-                t.bottom = ry + img.roi_h < t.ty + t.h ? static_cast<std::uint32_t>(ry + img.roi_h - t.ty) : t.h;
-                t.left = rx > t.tx ? static_cast<std::uint32_t>(rx - t.tx) : 0u;
-                t.right = rx + img.roi_w < t.tx + t.w ? static_cast<std::uint32_t>(rx + img.roi_w - t.tx) : t.w;
-                if constexpr (KIND == TSK_KIND_U16) {
-                    decode_tile<G, true, false, BLOCK>(img, t, stage, row_start);
-                } else if constexpr (KIND == TSK_KIND_F32) {
-                    decode_tile<G, false, false, BLOCK>(img, t, stage, row_start);
-                } else if constexpr (KIND == TSK_KIND_U16_CAL) {
-                    decode_tile<G, true, true, BLOCK>(img, t, stage, row_start);
-                } else if constexpr (KIND == TSK_KIND_F32_CAL) {
-                    decode_tile<G, false, true, BLOCK>(img, t, stage, row_start);
+            store = !(c.mode == 2 && c.rows == TSK_NO_ROWS) || walk_rows(c, u16, stage, row_start, &shared_flag);
+            if (!store) {
+                if (tid == 0) fail(a.results, c.image, kMalformed);
+            } else {
+                t = tile_view(c, img);
+                if constexpr (KIND == TSK_KIND_U16 || KIND == TSK_KIND_U16_CAL) {
+                    decode_tile_held<G, true, BLOCK, K>(img, t, stage, row_start, h);
+                } else if constexpr (KIND == TSK_KIND_F32 || KIND == TSK_KIND_F32_CAL) {
+                    decode_tile_held<G, false, BLOCK, K>(img, t, stage, row_start, h);
+                } else if (u16) {
+                    decode_tile_held<G, true, BLOCK, K>(img, t, stage, row_start, h);
                 } else {
-                    if (u16) {
-                        if (img.calibrated) decode_tile<G, true, true, BLOCK>(img, t, stage, row_start);
-                        else decode_tile<G, true, false, BLOCK>(img, t, stage, row_start);
-                    } else {
-                        if (img.calibrated) decode_tile<G, false, true, BLOCK>(img, t, stage, row_start);
-                        else decode_tile<G, false, false, BLOCK>(img, t, stage, row_start);
-                    }
+                    decode_tile_held<G, false, BLOCK, K>(img, t, stage, row_start, h);
                 }
+            }
+        }
+        if (more) await_prefetch(reg, row_reg, cursor.ahead);
+        if (store) {
+            if constexpr (KIND == TSK_KIND_U16) {
+                held_store<G, true, false, BLOCK, K>(img, t, h);
+            } else if constexpr (KIND == TSK_KIND_F32) {
+                held_store<G, false, false, BLOCK, K>(img, t, h);
+            } else if constexpr (KIND == TSK_KIND_U16_CAL) {
+                held_store<G, true, true, BLOCK, K>(img, t, h);
+            } else if constexpr (KIND == TSK_KIND_F32_CAL) {
+                held_store<G, false, true, BLOCK, K>(img, t, h);
+            } else if (img.type == TSK_UINT16) {
+                if (img.calibrated) held_store<G, true, true, BLOCK, K>(img, t, h);
+                else held_store<G, true, false, BLOCK, K>(img, t, h);
+            } else {
+                if (img.calibrated) held_store<G, false, true, BLOCK, K>(img, t, h);
+                else held_store<G, false, false, BLOCK, K>(img, t, h);
             }
         }
         if (!more) break;
     }
+}
+
+template <int G, int KIND, int BLOCK, int DEPTH>
+__device__ inline __attribute__((always_inline)) void decode_body(const tsk_decode_args &a) {
+    if constexpr (DEPTH > 1) decode_body_prefetch<G, KIND, BLOCK>(a);
+    else decode_body_staged<G, KIND, BLOCK>(a);
+}
+
+template <int G, int KIND, int BLOCK, int DEPTH>
+__global__ void __launch_bounds__(BLOCK) decode_kernel(tsk_decode_args a) {
+    decode_body<G, KIND, BLOCK, DEPTH>(a);
 }
 
 using DecodeFn = void (*)(tsk_decode_args);

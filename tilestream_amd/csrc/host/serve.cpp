@@ -166,15 +166,18 @@ Status ArchiveSet::info(std::uint32_t id, ArchiveInfo &out) const noexcept {
 namespace {
 
 // One batch in flight: a stream, the batch's tables staged in pinned host
-// memory and uploaded with one copy, and the event that marks its end.
+// memory and copied to the device by a kernel, the final results written back
+// into pinned host memory by a kernel, and the event that marks its end.
 //
 // Table layout (host and device alike): work counter | DeviceResult[n] |
 // tsk_decode_image[n] | tsk_request[n] | request index.
 struct Slot {
     hipStream_t stream = nullptr;
     hipEvent_t done = nullptr;
-    std::uint8_t *host = nullptr;
+    std::uint8_t *host = nullptr;      // tables, then the published results
+    std::uint8_t *host_view = nullptr; // the same memory as the device addresses it
     std::uint8_t *device = nullptr;
+    std::uint64_t published = 0;       // offset of the published results in `host`
     std::uint64_t first = 0, count = 0; // requests [first, first + count) of the current replay
     std::uint64_t bytes = 0;            // output bytes admitted
     bool busy = false;
@@ -263,10 +266,17 @@ Status Server::create(const ArchiveSet &archives, const ServerOptions &options, 
                 s.done = nullptr;
                 return detail::hip_failure();
             }
-            void *h = nullptr, *d = nullptr;
-            if (hipHostMalloc(&h, cap.total, hipHostMallocDefault) != hipSuccess) return detail::hip_failure();
+            void *h = nullptr, *hv = nullptr, *d = nullptr;
+            s.published = align_up(cap.total, 256);
+            const std::uint64_t host_bytes = s.published + std::uint64_t{options.max_batch} * sizeof(DeviceResult);
+            // Coherent: the device reads the tables and writes the results over
+            // the bus while the host polls.
+            if (hipHostMalloc(&h, host_bytes, hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess)
+                return detail::hip_failure();
             s.host = static_cast<std::uint8_t *>(h);
-            if (hipMalloc(&d, cap.total) != hipSuccess) return detail::hip_failure();
+            if (hipHostGetDevicePointer(&hv, h, 0) != hipSuccess) return detail::hip_failure();
+            s.host_view = static_cast<std::uint8_t *>(hv);
+            if (hipMalloc(&d, align_up(cap.total, 256)) != hipSuccess) return detail::hip_failure();
             s.device = static_cast<std::uint8_t *>(d);
         }
         server.reset(new Server(std::move(impl)));
@@ -389,12 +399,12 @@ Status Server::Impl::launch(Slot &slot, const RegionRequest *requests, std::uint
     args.request_count = static_cast<std::uint32_t>(count);
     args.requests = reinterpret_cast<const tsk_request *>(slot.device + l.requests);
     args.request_index = reinterpret_cast<const std::uint32_t *>(slot.device + l.index);
-    if (hipMemcpyAsync(slot.device, h, l.total, hipMemcpyHostToDevice, slot.stream) != hipSuccess ||
+    // Kernels read the staged tables from pinned memory and write the final
+    // results back into it: they start sooner than runtime copies.
+    if (tsk_copy_blocks(slot.device, slot.host_view, (l.total + 15) / 16, slot.stream) != hipSuccess ||
         tsk_decode(&args, slot.stream) != hipSuccess ||
-        tsk_results_complete(args.images, static_cast<std::uint32_t>(count), slot.device + l.results, slot.stream) !=
-            hipSuccess ||
-        hipMemcpyAsync(h + l.results, slot.device + l.results, count * sizeof(DeviceResult), hipMemcpyDeviceToHost,
-                       slot.stream) != hipSuccess ||
+        tsk_results_publish(args.images, static_cast<std::uint32_t>(count), slot.device + l.results,
+                            slot.host_view + slot.published, slot.stream) != hipSuccess ||
         hipEventRecord(slot.done, slot.stream) != hipSuccess)
         return detail::hip_failure();
     slot.first = first;
@@ -461,7 +471,7 @@ Status Server::replay(const RegionRequest *requests, const std::uint64_t *arriva
                     break;
                 }
                 const std::uint64_t t = elapsed();
-                const auto *res = reinterpret_cast<const DeviceResult *>(slot.host + kCounterBytes);
+                const auto *res = reinterpret_cast<const volatile DeviceResult *>(slot.host + slot.published);
                 for (std::uint64_t k = 0; k < slot.count; ++k) {
                     const std::uint64_t i = slot.first + k;
                     if (latency_ns != nullptr) latency_ns[i] = t - arrival_ns[i];
