@@ -222,6 +222,9 @@ def main() -> int:
     p.add_argument("--shape", default="8,256,8,2", help="group,block,chunk,depth used with --profile")
     p.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                    help="only run a plain copy kernel of this many MiB (scales HBM byte counters)")
+    p.add_argument("--pitch-align", type=int, default=256,
+                   help="output rows start at multiples of this many bytes, as hipMallocPitch "
+                        "allocates them (1: packed rows)")
     p.add_argument("--no-roi", action="store_true", help="skip the ROI serving leg")
     p.add_argument("--roi-only", action="store_true", help="only the ROI serving leg")
     p.add_argument("--roi-archives", type=int, default=64)
@@ -260,7 +263,7 @@ def main() -> int:
         return 0 if roi["verified_bit_exact"] in (None, True) else 3
     t0 = time.perf_counter()
     layout = generators.batch_layout(args.images, args.seed)
-    batch = ResidentBatch.build(layout, args.seed, threads)
+    batch = ResidentBatch.build(layout, args.seed, threads, pitch_align=args.pitch_align)
     log(f"built {len(layout)} images, {batch.decoded_bytes / 1e9:.2f} GB decoded, "
         f"{batch.archive_bytes / 1e9:.2f} GB archives in {time.perf_counter() - t0:.1f} s")
 
@@ -273,9 +276,10 @@ def main() -> int:
     st, queue = api.DecodeQueue.prepare(items, [api.Operation.decompress] * n, stream)
     assert st == api.Status.success, st
     qws = api.DeviceMemory(max(queue.workspace_bytes(), 256))
-    candidates = [(g, b, 0, c, d) for g in (4, 8, 16) for (b, d) in ((128, 1), (256, 1), (256, 2), (512, 2))
+    candidates = [(g, b, 0, c, d) for g in (4, 8, 16) for (b, d) in ((128, 1), (128, 2), (256, 2), (256, 3),
+                                                                    (512, 2), (512, 3))
                   for c in (4, 16)]
-    candidates += [(8, 256, 0, 0, 2), (4, 256, 0, 0, 2)]
+    candidates += [(8, 256, 0, 0, 2), (4, 256, 0, 0, 3)]
     if args.profile:
         grp, blk, chk, dep = (int(v) for v in args.shape.split(","))
         assert queue.set_shape(grp, blk, 0, chk, dep) == api.Status.success
@@ -295,6 +299,33 @@ def main() -> int:
         log(f"batched {ms_batched:.4f} ms per launch, "
             f"{batch.decoded_bytes / (ms_batched * 1e-3) / 1e9:.1f} GB/s decoded")
         return 0
+
+    # ---- the same launch into packed output rows (pitch = width x sample) -----------
+    packed = None
+    if args.pitch_align != 1:
+        from tilestream_amd.workloads import _align
+        offsets, total = [], 0
+        for e in batch.entries:
+            offsets.append(total)
+            total += _align(e.raw_bytes)
+        out2 = api.DeviceMemory(total)
+        items2 = []
+        for e, off in zip(batch.entries, offsets):
+            it = batch.item(batch.entries.index(e))
+            it.output = api.device_view(out2, e.ptype, e.width, e.height, e.width * e.sample, off,
+                                        capacity=e.raw_bytes)
+            items2.append(it)
+        st, q2 = api.DecodeQueue.prepare(items2, [api.Operation.decompress] * n, stream)
+        assert st == api.Status.success, st
+        assert q2.set_shape(best.group, best.block, best.grid, best.chunk, best.depth) == api.Status.success
+        assert q2.capture(qws, results, stream) == api.Status.success
+        for _ in range(args.warmup):
+            q2.launch(stream)
+        ms_packed = timed(stream, lambda: q2.launch(stream), args.steps)
+        packed = {"value": round(batch.decoded_bytes / (ms_packed * 1e-3) / 1e9, 2), "unit": "GB/s",
+                  "ms_per_step": round(ms_packed, 4), "note": "same launch shape, output rows packed"}
+        log(f"packed rows: {ms_packed:.4f} ms per launch")
+        del q2, out2, items2
 
     # ---- plain per-image path ------------------------------------------------------
     plans, ws_bytes = [], 0
@@ -356,7 +387,9 @@ def main() -> int:
                         f"float32 field, tiles from {{128x32, 64x64, 256x16}}, whole-image decode",
             "images": n, "tiles": int(stats.tasks), "decoded_bytes": decoded,
             "archive_bytes": batch.archive_bytes, "seed": args.seed,
+            "output_row_alignment_bytes": args.pitch_align,
         },
+        "packed_rows": packed,
         "per_image": {"value": round(gbps_plain, 2), "unit": "GB/s", "ms_per_step": round(ms_plain, 4),
                       "launches_per_step": 3 * n},
         "speedup_vs_per_image": round(ms_plain / ms_batched, 3),

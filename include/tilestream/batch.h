@@ -23,8 +23,8 @@ struct LaunchShape {
     std::uint32_t group = 8;   // columns decoded per lane: 1, 2, 4, 8, 16
     std::uint32_t block = 256; // threads per workgroup: 64, 128, 256, 512
     std::uint32_t grid = 0;    // workgroups; 0 = resident maximum of the device
-    std::uint32_t chunk = 8;   // tasks per draw from the work counter; 0 = static stride
-    std::uint32_t depth = 2;   // tiles a workgroup holds: 1 = staged only, 2 = next one prefetched
+    std::uint32_t chunk = 8;   // tasks per draw from the work counter (last eighth); 0 = static stride only
+    std::uint32_t depth = 2;   // tiles a workgroup holds: 1 = staged only, 2 or 3 = 1 or 2 more in flight
 };
 
 struct QueueStats {

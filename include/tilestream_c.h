@@ -117,7 +117,8 @@ uint32_t ts_synchronize(void *stream);
 
 /* ---- extension: launch shape of the decode kernel ---------------------- */
 /* group: columns per lane (1,2,4,8,16); block: 64..512; grid: 0 = one workgroup per tile;
- * depth: 1 staged, 2 next tile prefetched.  invalid_argument for a combination without a kernel. */
+ * depth: 1 staged; 2 or 3: one or two more tiles in flight.  invalid_argument for a combination
+ * without a kernel. */
 uint32_t ts_plan_set_decode_shape(ts_plan *plan, uint32_t group, uint32_t block, uint32_t grid, uint32_t depth);
 
 /* ---- extension: batched persistent decode (tilestream/batch.h) ---------
@@ -190,9 +191,9 @@ int ts_hip_malloc(uint64_t bytes, void **data);
 int ts_hip_free(void *data);
 int ts_hip_host_malloc(uint64_t bytes, void **data); /* pinned, mapped */
 int ts_hip_host_free(void *data);
-int ts_hip_memcpy(void *dst, const void *src, uint64_t bytes);  /* any direction, synchronous */
+int ts_hip_memcpy(void *dst, const void *src, uint64_t bytes);  /* any direction; complete on return */
 int ts_hip_memcpy_async(void *dst, const void *src, uint64_t bytes, void *stream);
-int ts_hip_memset(void *dst, int value, uint64_t bytes);
+int ts_hip_memset(void *dst, int value, uint64_t bytes);          /* complete on return */
 int ts_hip_stream_create(void **stream);
 int ts_hip_stream_destroy(void *stream);
 int ts_hip_stream_synchronize(void *stream);

@@ -102,8 +102,8 @@ typedef struct {
     uint32_t group;              /* columns per lane: 1, 2, 4, 8 or 16 */
     uint32_t block;              /* threads per workgroup: 64, 128, 256 or 512 */
     uint32_t grid;               /* workgroups; tiles are strided or drawn from the queue */
-    uint32_t chunk;              /* tiles taken per queue draw */
-    uint32_t depth;              /* tiles a workgroup holds at once: 1 staged, 2 = + next prefetched */
+    uint32_t chunk;              /* tiles per queue draw (the last eighth of the tasks; the rest by stride) */
+    uint32_t depth;              /* tiles a workgroup holds at once: 1 staged; 2, 3: 1 or 2 more in flight */
     uint32_t request_count;      /* request mode when non-zero (requests, request_index set) */
     const tsk_request *requests;
     const uint32_t *request_index; /* task / TSK_REQUEST_STRIDE -> request */

@@ -32,7 +32,7 @@ def test_whole_image_shapes(orc, tile, kind):
     assert np.array_equal(out, px)
 
 
-SHAPE_GRID = [(g_, b_, d_) for g_ in (1, 2, 4, 8, 16) for b_ in (64, 128, 256, 512) for d_ in (1, 2)]
+SHAPE_GRID = [(g_, b_, d_) for g_ in (1, 2, 4, 8, 16) for b_ in (64, 128, 256, 512) for d_ in (1, 2, 3)]
 
 
 @pytest.mark.parametrize("group,block,depth", SHAPE_GRID)

@@ -51,7 +51,8 @@ def _run_queue(batch, ops, rois, shape=None, stream=None, graph_replays=0, cal=N
 def test_whole_batch_matches_pixels(batch):
     n = len(batch.entries)
     for shape in [(8, 256, 0, 1, 2), (4, 128, 0, 4, 1), (16, 512, 0, 2, 2), (8, 256, 0, 0, 2),
-                  (2, 256, 37, 1, 1), (4, 512, 5, 3, 2), (8, 64, 0, 16, 1)]:
+                  (2, 256, 37, 1, 1), (4, 512, 5, 3, 2), (8, 64, 0, 16, 1), (4, 256, 0, 4, 3),
+                  (8, 512, 7, 2, 3), (16, 256, 0, 0, 3), (4, 128, 0, 4, 2)]:
         batch.clear_outputs()
         ops = [ts.Operation.decompress] * n
         rois = [(0, 0, e.width, e.height) for e in batch.entries]

@@ -92,7 +92,8 @@ def _same(got, want):
     return bool(np.array_equal(got, want))
 
 
-@pytest.mark.parametrize("shape", [(8, 256, 0, 4, 2), (4, 128, 0, 0, 1), (16, 512, 0, 2, 2), (8, 256, 3, 1, 1)])
+@pytest.mark.parametrize("shape", [(8, 256, 0, 4, 2), (4, 128, 0, 0, 1), (16, 512, 0, 2, 2), (8, 256, 3, 1, 1),
+                                   (4, 256, 0, 4, 3), (8, 512, 0, 1, 3)])
 def test_batch_matches_oracle(orc, resident, shape):
     rng = np.random.default_rng(500 + shape[0] + shape[1])
     specs = _requests(resident, rng, 90)

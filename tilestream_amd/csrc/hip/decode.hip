@@ -140,7 +140,7 @@ int tsk_preload_decode(std::uint32_t calibrated) {
     for (std::uint32_t kind : kinds)
         for (std::uint32_t g : {1u, 2u, 4u, 8u, 16u})
             for (std::uint32_t b : {64u, 128u, 256u, 512u})
-                for (std::uint32_t d : {1u, 2u}) {
+                for (std::uint32_t d : {1u, 2u, 3u}) {
                     const DecodeFn fn = decode_function(kind, g, b, d);
                     if (fn == nullptr) continue;
                     const hipError_t e = hipFuncGetAttributes(&attr, reinterpret_cast<const void *>(fn));

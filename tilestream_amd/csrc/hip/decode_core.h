@@ -489,11 +489,9 @@ __device__ inline tsk_decode_task request_task(const tsk_decode_args &a, std::ui
 
 // Reads a task descriptor and, on the trusted path, the producer's result and
 // the device-resident index entry (reference decoder.cu:191-220).
-__device__ inline Pending describe(const tsk_decode_args &a, std::uint64_t i) {
-    std::uint32_t request = 0;
-    const tsk_decode_task task = a.request_count != 0 ? request_task(a, i, request) : load_constant(a.tasks, i);
+__device__ inline Pending describe_task(const tsk_decode_args &a, const tsk_decode_task &task, std::uint32_t image) {
     Pending p;
-    p.image = a.request_count != 0 ? request : task.image;
+    p.image = image;
     p.column = task.column;
     p.row = task.row;
     p.rows = task.rows;
@@ -539,18 +537,54 @@ __device__ inline Pending describe(const tsk_decode_args &a, std::uint64_t i) {
     return p;
 }
 
-// Work assignment: a static stride, or chunks drawn from a device counter with
-// the next chunk requested one chunk ahead.
+__device__ inline Pending describe(const tsk_decode_args &a, std::uint64_t i) {
+    if (a.request_count != 0) {
+        std::uint32_t request = 0;
+        const tsk_decode_task task = request_task(a, i, request);
+        return describe_task(a, task, request);
+    }
+    const tsk_decode_task task = load_constant(a.tasks, i);
+    return describe_task(a, task, task.image);
+}
+
+// Work assignment.  Without a work counter: a static stride over the tasks.
+// With one: the first seven eighths of the tasks by static stride (every
+// workgroup busy on neighbouring tasks, so payload reads and output writes
+// stay local, with no atomics), the rest in chunks drawn from the counter
+// (the next chunk requested one chunk ahead) to even out the finish.
 struct Cursor {
-    std::uint64_t next, end, ahead;
+    std::uint64_t next, end;
+    std::uint64_t ahead; // the chunk requested one chunk early (thread 0)
+    bool dynamic;
 };
+
+__device__ inline std::uint64_t static_end(const tsk_decode_args &a) { return a.task_count - a.task_count / 8; }
+
+// The stride position of this workgroup.  Workgroups are dispatched to the
+// eight XCDs (each with its own L2) round robin, so workgroup b runs on XCD
+// b % 8.  Giving each XCD a contiguous run of grid / 8 neighbouring tasks per
+// stride step keeps neighbouring tiles, whose output rows share cache lines
+// at tile edges, in one L2, which then writes those lines back whole.
+__device__ inline std::uint64_t stride_slot() {
+    const std::uint32_t b = blockIdx.x, g = gridDim.x;
+    return g % 8 == 0 ? std::uint64_t{b % 8} * (g / 8) + b / 8 : b;
+}
+
+// Thread 0 requests the first dynamic chunk at kernel start.
+__device__ inline void queue_start(const tsk_decode_args &a, Cursor &c) {
+    if (threadIdx.x == 0) c.ahead = atomicAdd(reinterpret_cast<unsigned long long *>(a.queue), a.chunk);
+}
 
 template <int BLOCK>
 __device__ inline std::uint64_t draw(const tsk_decode_args &a, Cursor &c, std::uint64_t *shared_next) {
-    if (a.queue == nullptr) {
-        const std::uint64_t i = c.next;
-        c.next += gridDim.x;
-        return i;
+    if (a.queue == nullptr || !c.dynamic) {
+        if (a.queue == nullptr || c.next < static_end(a)) {
+            const std::uint64_t i = c.next;
+            c.next += gridDim.x;
+            return i;
+        }
+        c.dynamic = true;
+        c.next = c.end = 0;
     }
     if (c.next >= c.end) {
         // Publish the chunk requested earlier and request the one after it.
@@ -562,7 +596,7 @@ __device__ inline std::uint64_t draw(const tsk_decode_args &a, Cursor &c, std::u
         __syncthreads();
         // The chunk start is the same for every lane: keep it (and every
         // descriptor read through it) in scalar registers.
-        const std::uint64_t start = shared_next[0];
+        const std::uint64_t start = shared_next[0] + static_end(a);
         c.next = (static_cast<std::uint64_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(start >> 32))) << 32) |
                  static_cast<std::uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(start)));
         c.end = c.next + a.chunk;
@@ -627,25 +661,29 @@ __device__ inline void await_prefetch(const u32x4 (&reg)[KMAX], std::uint32_t ro
                  : "memory");
 }
 
-// Loads a tile's staged blocks and row offsets into registers.  Every lane
-// loads every slot, clamped to the tile's last block (or to the first block
-// of the task table for a tile without blocks), so that no load is
-// conditional: the register array then stays in place instead of being
-// copied at each branch.
+// Loads a tile's staged blocks and row offsets into registers.  A slot is
+// loaded when any lane of the workgroup needs it (a uniform branch); within
+// it, lanes past the tile's last block load that block again, so no load is
+// masked per lane and the register array stays in place.
 template <int BLOCK, int KMAX>
 __device__ inline void prefetch(const tsk_decode_args &a, const Pending &p, u32x4 (&reg)[KMAX], std::uint32_t &row_reg) {
-    const gptr<const u32x4> base =
-        p.nblocks != 0 ? p.blocks : reinterpret_cast<gptr<const u32x4>>(reinterpret_cast<std::uintptr_t>(a.tasks));
+    const gptr<const u32x4> base = p.blocks;
     const std::uint32_t last = p.nblocks != 0 ? p.nblocks - 1 : 0u;
 #pragma unroll
     for (int k = 0; k < KMAX; ++k) {
         const std::uint32_t b = threadIdx.x + k * BLOCK;
-        reg[k] = __builtin_nontemporal_load(base + (b < last ? b : last));
+        if (k * BLOCK < p.nblocks) reg[k] = __builtin_nontemporal_load(base + (b < last ? b : last));
     }
+    // The row offsets load is unconditional too (from the task table when the
+    // tile has none), so every prefetch issues the same number of loads.
     const bool rows = p.mode == 2 && p.rows != TSK_NO_ROWS && p.rows < a.row_count;
     const std::uint64_t r = rows ? p.rows + (threadIdx.x < p.h ? threadIdx.x : 0u) : 0u;
     const std::uint64_t r_clamped = r < a.row_count ? r : p.rows;
-    row_reg = rows ? (global(a.rows) + r_clamped)[0] : 0u;
+    const gptr<const std::uint16_t> rp =
+        rows ? global(a.rows) + r_clamped
+             : reinterpret_cast<gptr<const std::uint16_t>>(reinterpret_cast<std::uintptr_t>(a.tasks));
+    const std::uint32_t v = rp[0];
+    row_reg = rows ? v : 0u;
 }
 
 // Depth 1: the payload of a tile is loaded after the previous one is decoded.
@@ -657,11 +695,8 @@ __device__ inline __attribute__((always_inline)) void decode_body_staged(const t
     __shared__ std::uint64_t shared_next[1];
 
     const std::uint32_t tid = threadIdx.x;
-    Cursor cursor{blockIdx.x, 0, 0};
-    if (a.queue != nullptr) {
-        if (tid == 0) cursor.ahead = atomicAdd(reinterpret_cast<unsigned long long *>(a.queue), a.chunk);
-        cursor.next = cursor.end = 0;
-    }
+    Cursor cursor{stride_slot(), 0, 0, false};
+    if (a.queue != nullptr) queue_start(a, cursor);
     for (;;) {
         const std::uint64_t i = draw<BLOCK>(a, cursor, shared_next);
         if (i >= a.task_count) return;
@@ -702,102 +737,167 @@ __device__ inline __attribute__((always_inline)) void decode_body_staged(const t
     }
 }
 
-// Depth 2: while a tile is decoded into registers the next tile's payload is
-// loaded into registers; after the wait for it, the decoded groups are stored
-// and the payload is committed to LDS for the next round.
-template <int G, int KIND, int BLOCK>
-__device__ inline __attribute__((always_inline)) void decode_body_prefetch(const tsk_decode_args &a) {
-    __shared__ __attribute__((aligned(16))) std::uint32_t stage[kStageWords];
-    __shared__ std::uint16_t row_start[64];
-    __shared__ std::uint32_t shared_flag;
-    __shared__ std::uint64_t shared_next[1];
-    constexpr int KMAX = static_cast<int>((kMaxBlocks + BLOCK - 1) / BLOCK);
-    constexpr int K = held_groups<G, BLOCK>();
-
-    const std::uint32_t tid = threadIdx.x;
-    Cursor cursor{blockIdx.x, 0, 0};
-    if (a.queue != nullptr) {
-        if (tid == 0) cursor.ahead = atomicAdd(reinterpret_cast<unsigned long long *>(a.queue), a.chunk);
-        cursor.next = cursor.end = 0;
+// Decodes staged tile c into registers; false when nothing is to be stored.
+template <int G, int KIND, int BLOCK, int K>
+__device__ inline __attribute__((always_inline)) bool decode_phase(const tsk_decode_args &a, const Pending &c,
+                                                                   const std::uint32_t *stage, std::uint16_t *row_start,
+                                                                   std::uint32_t *shared_flag, tsk_decode_image &img,
+                                                                   TileView &t, Held<G, K> &h) {
+    if (c.state == 2) {
+        if (threadIdx.x == 0) fail(a.results, c.image, kMalformed);
+        return false;
     }
-    const std::uint64_t i = draw<BLOCK>(a, cursor, shared_next);
-    if (i >= a.task_count) return;
+    if (c.state != 0) return false;
+    // The descriptor is copied by value: output stores cannot alias it.
+    img = load_constant(a.images, c.image);
+    const bool u16 = img.type == TSK_UINT16;
+    if (c.mode == 2 && c.rows == TSK_NO_ROWS && !walk_rows(c, u16, stage, row_start, shared_flag)) {
+        if (threadIdx.x == 0) fail(a.results, c.image, kMalformed);
+        return false;
+    }
+    t = tile_view(c, img);
+    if constexpr (KIND == TSK_KIND_U16 || KIND == TSK_KIND_U16_CAL) {
+        decode_tile_held<G, true, BLOCK, K>(img, t, stage, row_start, h);
+    } else if constexpr (KIND == TSK_KIND_F32 || KIND == TSK_KIND_F32_CAL) {
+        decode_tile_held<G, false, BLOCK, K>(img, t, stage, row_start, h);
+    } else if (u16) {
+        decode_tile_held<G, true, BLOCK, K>(img, t, stage, row_start, h);
+    } else {
+        decode_tile_held<G, false, BLOCK, K>(img, t, stage, row_start, h);
+    }
+    return true;
+}
 
-    Pending p = describe(a, i);
-    u32x4 reg[KMAX];
-    std::uint32_t row_reg = 0;
-    prefetch<BLOCK>(a, p, reg, row_reg);
-    await_prefetch(reg, row_reg, cursor.ahead);
+template <int G, int KIND, int BLOCK, int K>
+__device__ inline __attribute__((always_inline)) void store_phase(const tsk_decode_image &img, const TileView &t,
+                                                                  const Held<G, K> &h) {
+    if constexpr (KIND == TSK_KIND_U16) {
+        held_store<G, true, false, BLOCK, K>(img, t, h);
+    } else if constexpr (KIND == TSK_KIND_F32) {
+        held_store<G, false, false, BLOCK, K>(img, t, h);
+    } else if constexpr (KIND == TSK_KIND_U16_CAL) {
+        held_store<G, true, true, BLOCK, K>(img, t, h);
+    } else if constexpr (KIND == TSK_KIND_F32_CAL) {
+        held_store<G, false, true, BLOCK, K>(img, t, h);
+    } else if (img.type == TSK_UINT16) {
+        if (img.calibrated) held_store<G, true, true, BLOCK, K>(img, t, h);
+        else held_store<G, true, false, BLOCK, K>(img, t, h);
+    } else {
+        if (img.calibrated) held_store<G, false, true, BLOCK, K>(img, t, h);
+        else held_store<G, false, false, BLOCK, K>(img, t, h);
+    }
+}
 
-    for (;;) {
-        const Pending c = p;
-        __syncthreads(); // the previous tile is done with the LDS
+template <int BLOCK, int KMAX>
+__device__ inline void commit(const tsk_decode_args &a, const Pending &q, const u32x4 (&reg)[KMAX], std::uint32_t row_reg,
+                              std::uint32_t *stage, std::uint16_t *row_start) {
+    const std::uint32_t tid = threadIdx.x;
 #pragma unroll
-        for (int k = 0; k < KMAX; ++k)
-            if (tid + k * BLOCK < c.nblocks) *reinterpret_cast<u32x4 *>(stage + 4 * (tid + k * BLOCK)) = reg[k];
-        if (c.mode == 2 && c.rows != TSK_NO_ROWS && tid < c.h && c.rows + tid < a.row_count)
-            row_start[tid] = static_cast<std::uint16_t>(row_reg);
-        __syncthreads();
+    for (int k = 0; k < KMAX; ++k)
+        if (tid + k * BLOCK < q.nblocks) *reinterpret_cast<u32x4 *>(stage + 4 * (tid + k * BLOCK)) = reg[k];
+    if (q.mode == 2 && q.rows != TSK_NO_ROWS && tid < q.h && q.rows + tid < a.row_count)
+        row_start[tid] = static_cast<std::uint16_t>(row_reg);
+}
 
-        // Start fetching the next tile; its loads are in flight while this one decodes.
+// Depth 2 and 3: two LDS stage buffers and DEPTH - 1 register sets.  Tile n
+// is decoded (into registers) from one buffer while the payloads of the next
+// DEPTH - 1 tiles are in flight into registers.  Then the oldest set is
+// awaited (the wait covers only loads issued before it and the previous
+// tile's stores), committed to the other buffer, tile n's groups are stored,
+// the set is reloaded with the tile DEPTH - 1 ahead, and one barrier ends the
+// round.
+template <int G, int KIND, int BLOCK, int DEPTH>
+struct Pipeline {
+    static constexpr int KMAX = static_cast<int>((kMaxBlocks + BLOCK - 1) / BLOCK);
+    static constexpr int K = held_groups<G, BLOCK>();
+    static constexpr int SETS = DEPTH - 1;
+
+    const tsk_decode_args &a;
+    std::uint32_t (*stage)[kStageWords];
+    std::uint16_t (*row_start)[64];
+    std::uint32_t *shared_flag;
+    std::uint64_t *shared_next;
+    Cursor cursor;
+    u32x4 reg[SETS][KMAX];
+    std::uint32_t row_reg[SETS];
+    Pending c;         // staged in stage[buf]
+    Pending p[SETS];   // p[0]: next tile, in flight in set `next`; p[1]: the one after
+    bool more[SETS];
+    std::uint32_t buf;
+
+    __device__ inline __attribute__((always_inline)) void load(int set, std::uint32_t slot) {
         const std::uint64_t n = draw<BLOCK>(a, cursor, shared_next);
-        const bool more = n < a.task_count;
-        if (more) {
-            p = describe(a, n);
-            prefetch<BLOCK>(a, p, reg, row_reg);
-        }
+        more[slot] = n < a.task_count;
+        if (more[slot]) p[slot] = describe(a, n);
+        else p[slot].nblocks = 0, p[slot].mode = 0;
+        prefetch<BLOCK>(a, p[slot], reg[set], row_reg[set]);
+    }
 
-        bool store = false;
+    // One round with the next tile's payload in register set S.
+    template <int S> __device__ inline __attribute__((always_inline)) bool round() {
         tsk_decode_image img{};
         TileView t{};
         Held<G, K> h;
-        if (c.state == 2) {
-            if (tid == 0) fail(a.results, c.image, kMalformed);
-        } else if (c.state == 0) {
-            // The descriptor is copied by value: output stores cannot alias it.
-            img = load_constant(a.images, c.image);
-            const bool u16 = img.type == TSK_UINT16;
-            store = !(c.mode == 2 && c.rows == TSK_NO_ROWS) || walk_rows(c, u16, stage, row_start, &shared_flag);
-            if (!store) {
-                if (tid == 0) fail(a.results, c.image, kMalformed);
-            } else {
-                t = tile_view(c, img);
-                if constexpr (KIND == TSK_KIND_U16 || KIND == TSK_KIND_U16_CAL) {
-                    decode_tile_held<G, true, BLOCK, K>(img, t, stage, row_start, h);
-                } else if constexpr (KIND == TSK_KIND_F32 || KIND == TSK_KIND_F32_CAL) {
-                    decode_tile_held<G, false, BLOCK, K>(img, t, stage, row_start, h);
-                } else if (u16) {
-                    decode_tile_held<G, true, BLOCK, K>(img, t, stage, row_start, h);
-                } else {
-                    decode_tile_held<G, false, BLOCK, K>(img, t, stage, row_start, h);
-                }
-            }
+        const bool store = decode_phase<G, KIND, BLOCK, K>(a, c, stage[buf], row_start[buf], shared_flag, img, t, h);
+        const bool had_next = more[0];
+        if (had_next) {
+            await_prefetch(reg[S], row_reg[S], cursor.ahead);
+            commit<BLOCK>(a, p[0], reg[S], row_reg[S], stage[buf ^ 1u], row_start[buf ^ 1u]);
         }
-        if (more) await_prefetch(reg, row_reg, cursor.ahead);
-        if (store) {
-            if constexpr (KIND == TSK_KIND_U16) {
-                held_store<G, true, false, BLOCK, K>(img, t, h);
-            } else if constexpr (KIND == TSK_KIND_F32) {
-                held_store<G, false, false, BLOCK, K>(img, t, h);
-            } else if constexpr (KIND == TSK_KIND_U16_CAL) {
-                held_store<G, true, true, BLOCK, K>(img, t, h);
-            } else if constexpr (KIND == TSK_KIND_F32_CAL) {
-                held_store<G, false, true, BLOCK, K>(img, t, h);
-            } else if (img.type == TSK_UINT16) {
-                if (img.calibrated) held_store<G, true, true, BLOCK, K>(img, t, h);
-                else held_store<G, true, false, BLOCK, K>(img, t, h);
-            } else {
-                if (img.calibrated) held_store<G, false, true, BLOCK, K>(img, t, h);
-                else held_store<G, false, false, BLOCK, K>(img, t, h);
-            }
+        if (store) store_phase<G, KIND, BLOCK, K>(img, t, h);
+        if (!had_next) return false;
+        c = p[0];
+        if constexpr (SETS == 2) {
+            p[0] = p[1];
+            more[0] = more[1];
+            load(S, 1);
+        } else {
+            load(S, 0);
         }
-        if (!more) break;
+        __syncthreads(); // the committed tile is visible; the decoded buffer is free
+        buf ^= 1u;
+        return true;
     }
+
+    __device__ inline __attribute__((always_inline)) void run() {
+        if (a.queue != nullptr) queue_start(a, cursor);
+        const std::uint64_t i = draw<BLOCK>(a, cursor, shared_next);
+        if (i >= a.task_count) return;
+        c = describe(a, i);
+        prefetch<BLOCK>(a, c, reg[0], row_reg[0]);
+        await_prefetch(reg[0], row_reg[0], cursor.ahead);
+        commit<BLOCK>(a, c, reg[0], row_reg[0], stage[0], row_start[0]);
+        buf = 0;
+        if constexpr (SETS == 2) {
+            load(1, 0);
+            load(0, 1);
+            __syncthreads();
+            for (;;) {
+                if (!round<1>()) break;
+                if (!round<0>()) break;
+            }
+        } else {
+            load(0, 0);
+            __syncthreads();
+            while (round<0>()) {
+            }
+        }
+    }
+};
+
+template <int G, int KIND, int BLOCK, int DEPTH>
+__device__ inline __attribute__((always_inline)) void decode_body_prefetch(const tsk_decode_args &a) {
+    __shared__ __attribute__((aligned(16))) std::uint32_t stage[2][kStageWords];
+    __shared__ std::uint16_t row_start[2][64];
+    __shared__ std::uint32_t shared_flag;
+    __shared__ std::uint64_t shared_next[1];
+    Pipeline<G, KIND, BLOCK, DEPTH> pipe{a, stage, row_start, &shared_flag, shared_next, Cursor{stride_slot(), 0, 0, false}};
+    pipe.run();
 }
 
 template <int G, int KIND, int BLOCK, int DEPTH>
 __device__ inline __attribute__((always_inline)) void decode_body(const tsk_decode_args &a) {
-    if constexpr (DEPTH > 1) decode_body_prefetch<G, KIND, BLOCK>(a);
+    if constexpr (DEPTH > 1) decode_body_prefetch<G, KIND, BLOCK, DEPTH>(a);
     else decode_body_staged<G, KIND, BLOCK>(a);
 }
 
@@ -809,15 +909,16 @@ __global__ void __launch_bounds__(BLOCK) decode_kernel(tsk_decode_args a) {
 using DecodeFn = void (*)(tsk_decode_args);
 
 // The (group, block, depth) combinations that are compiled: every group with
-// 256 threads and depth 1; groups 4, 8, 16 with every block size and depth,
-// depth 2 only for 256 and 512 threads (its register prefetch holds
-// ceil(1025 / block) 16-byte blocks per thread).
+// 256 threads and depth 1; groups 4, 8, 16 with every block size at depth 1,
+// 128, 256 and 512 threads at depth 2 and 256 and 512 threads at depth 3
+// (each register set holds ceil(1025 / block) 16-byte blocks per thread).
 constexpr bool compiled(std::uint32_t group, std::uint32_t block, std::uint32_t depth) {
     const bool g_ok = group == 1 || group == 2 || group == 4 || group == 8 || group == 16;
     const bool b_ok = block == 64 || block == 128 || block == 256 || block == 512;
-    if (!g_ok || !b_ok || depth < 1 || depth > 2) return false;
+    if (!g_ok || !b_ok || depth < 1 || depth > 3) return false;
     if (group <= 2) return block == 256 && depth == 1;
-    return depth == 1 || block >= 256;
+    if (depth == 3) return block >= 256;
+    return depth == 1 || block >= 128;
 }
 
 template <int KIND, int G> DecodeFn pick_block(std::uint32_t block, std::uint32_t depth) {
@@ -835,8 +936,16 @@ template <int KIND, int G> DecodeFn pick_block(std::uint32_t block, std::uint32_
         }
         if (depth == 2) {
             switch (block) {
+            case 128: return decode_kernel<G, KIND, 128, 2>;
             case 256: return decode_kernel<G, KIND, 256, 2>;
             case 512: return decode_kernel<G, KIND, 512, 2>;
+            default: return nullptr;
+            }
+        }
+        if (depth == 3) {
+            switch (block) {
+            case 256: return decode_kernel<G, KIND, 256, 3>;
+            case 512: return decode_kernel<G, KIND, 512, 3>;
             default: return nullptr;
             }
         }

@@ -361,11 +361,22 @@ int ts_hip_malloc(uint64_t bytes, void **data) { return hipMalloc(data, bytes); 
 int ts_hip_free(void *data) { return hipFree(data); }
 int ts_hip_host_malloc(uint64_t bytes, void **data) { return hipHostMalloc(data, bytes, hipHostMallocMapped); }
 int ts_hip_host_free(void *data) { return hipHostFree(data); }
-int ts_hip_memcpy(void *dst, const void *src, uint64_t bytes) { return hipMemcpy(dst, src, bytes, hipMemcpyDefault); }
+// Both helpers complete before returning.  hipMemset and device-to-device
+// hipMemcpy may still run on the null stream after they return, and the null
+// stream does not order work on streams created non-blocking.
+int ts_hip_memcpy(void *dst, const void *src, uint64_t bytes) {
+    hipError_t e = hipMemcpy(dst, src, bytes, hipMemcpyDefault);
+    if (e == hipSuccess) e = hipStreamSynchronize(nullptr);
+    return e;
+}
 int ts_hip_memcpy_async(void *dst, const void *src, uint64_t bytes, void *stream) {
     return hipMemcpyAsync(dst, src, bytes, hipMemcpyDefault, stream_of(stream));
 }
-int ts_hip_memset(void *dst, int value, uint64_t bytes) { return hipMemset(dst, value, bytes); }
+int ts_hip_memset(void *dst, int value, uint64_t bytes) {
+    hipError_t e = hipMemset(dst, value, bytes);
+    if (e == hipSuccess) e = hipStreamSynchronize(nullptr);
+    return e;
+}
 int ts_hip_stream_create(void **stream) {
     hipStream_t s = nullptr;
     const hipError_t e = hipStreamCreateWithFlags(&s, hipStreamNonBlocking);

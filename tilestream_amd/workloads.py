@@ -37,6 +37,7 @@ class Entry:
     meta: Optional[api.Metadata] = None
     src_offset: int = 0
     out_offset: int = 0
+    out_pitch: int = 0
 
     @property
     def ptype(self):
@@ -59,8 +60,10 @@ class ResidentBatch:
     keep: list = field(default_factory=list)
 
     @classmethod
-    def build(cls, layout, seed: int, threads: Optional[int] = None, keep_pixels: bool = True):
-        """layout: iterable of (width, height, dtype, tile)."""
+    def build(cls, layout, seed: int, threads: Optional[int] = None, keep_pixels: bool = True,
+              pitch_align: int = 1):
+        """layout: iterable of (width, height, dtype, tile).  Output rows start
+        every width * sample bytes rounded up to `pitch_align`."""
         entries = [Entry(w, h, dt, tile, seed + i) for i, (w, h, dt, tile) in enumerate(layout)]
         threads = threads or max(1, min(16, os.cpu_count() or 1))
 
@@ -81,7 +84,8 @@ class ResidentBatch:
             e.src_offset = src_total
             src_total += _align(len(e.archive))
             e.out_offset = out_total
-            out_total += _align(e.raw_bytes)
+            e.out_pitch = (e.width * e.sample + pitch_align - 1) // pitch_align * pitch_align
+            out_total += _align(e.out_pitch * (e.height - 1) + e.width * e.sample)
             st, e.meta = api.parse_metadata(e.archive)
             if st != api.Status.success:
                 raise RuntimeError(f"parse failed: {st}")
@@ -106,8 +110,10 @@ class ResidentBatch:
         e = self.entries[i]
         it = BatchItem()
         x, y, w, h = roi if roi is not None else (0, 0, e.width, e.height)
-        it.output = api.device_view(self.out, e.ptype, w, h, w * e.sample, e.out_offset,
-                                    capacity=_align(e.raw_bytes))
+        full = roi is None or (w, h) == (e.width, e.height)
+        pitch = e.out_pitch if full else w * e.sample
+        it.output = api.device_view(self.out, e.ptype, w, h, pitch, e.out_offset,
+                                    capacity=pitch * (h - 1) + w * e.sample)
         it.compressed_input.bytes = ConstBytes(self.src.ptr + e.src_offset, len(e.archive))
         it.compressed_input.metadata = e.meta.handle
         it.roi = Roi(x, y, w, h)
@@ -115,8 +121,11 @@ class ResidentBatch:
 
     def read_output(self, i: int) -> np.ndarray:
         e = self.entries[i]
-        raw = self.out.download(e.raw_bytes, e.out_offset)
-        return raw.view(np.uint16 if e.sample == 2 else np.uint32).reshape(e.height, e.width)
+        span = e.out_pitch * (e.height - 1) + e.width * e.sample
+        raw = np.zeros(e.out_pitch * e.height, np.uint8)
+        raw[:span] = self.out.download(span, e.out_offset)
+        rows = raw.reshape(e.height, e.out_pitch)[:, : e.width * e.sample]
+        return np.ascontiguousarray(rows).view(np.uint16 if e.sample == 2 else np.uint32)
 
     def clear_outputs(self) -> None:
         self.out.fill(0)
