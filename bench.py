@@ -109,20 +109,36 @@ def roi_leg(args, threads: int, stream):
         assert st == api.Status.success, st
     assert aset.commit(stream) == api.Status.success
     rois = generators.roi_requests(args.roi_requests, args.seed + 5, side, side, archives=args.roi_archives)
-    offsets, total = [], 0
+    align = args.pitch_align
+    offsets, pitches, total = [], [], 0
     for a, x, y, w, h in rois:
+        pitch = (w * 4 + align - 1) // align * align
         offsets.append(total)
-        total += (w * h * 4 + 255) // 256 * 256
+        pitches.append(pitch)
+        total += (pitch * (h - 1) + w * 4 + 255) // 256 * 256
     out = api.DeviceMemory(total)
     requests = []
-    for (a, x, y, w, h), off in zip(rois, offsets):
-        view = api.device_view(out, api.PixelType.float32, w, h, offset=off, capacity=w * h * 4)
+    for (a, x, y, w, h), off, pitch in zip(rois, offsets, pitches):
+        view = api.device_view(out, api.PixelType.float32, w, h, pitch, offset=off,
+                               capacity=pitch * (h - 1) + w * 4)
         requests.append(api.region_request(a, (x, y, w, h), view, True, offset, gain))
+    # Algorithmic bytes: the payload of every tile a region touches (whole tiles
+    # are decoded) plus the FP32 region written.
+    lengths = []
+    for e in rb.entries:
+        meta = e.meta
+        cols, rows_ = (side + tile[0] - 1) // tile[0], (side + tile[1] - 1) // tile[1]
+        lengths.append(np.array([meta.tile(k).length for k in range(cols * rows_)], np.int64).reshape(rows_, cols))
+    touched = 0
+    for a, x, y, w, h in rois:
+        touched += int(lengths[a][y // tile[1]:(y + h - 1) // tile[1] + 1, x // tile[0]:(x + w - 1) // tile[0] + 1].sum())
     decoded = sum(w * h * 2 for _, _, _, w, h in rois)
     log(f"roi: {len(rb.entries)} archives ({rb.archive_bytes / 1e9:.2f} GB), {len(rois)} requests, "
         f"{decoded / 1e9:.2f} GB decoded, {total / 1e9:.2f} GB FP32 out, set up in {time.perf_counter() - t0:.1f} s")
 
-    server = api.Server(aset, api.server_options(streams=args.roi_streams, max_batch=args.roi_max_batch))
+    shape = tuple(int(v) for v in args.roi_shape.split(","))
+    server = api.Server(aset, api.server_options(streams=args.roi_streams, max_batch=args.roi_max_batch,
+                                                 shape=shape))
     n = len(requests)
     # Saturation: every request arrives at once; continuous batching drains them.
     sat = None
@@ -187,7 +203,10 @@ def roi_leg(args, threads: int, stream):
         for i in sorted(set(range(0, n, max(1, n // 200))) | {big, n - 1}):
             a, x, y, w, h = rois[i]
             want = ((rb.entries[a].pixels[y:y + h, x:x + w].astype(np.float32) - o32) * g32).view(np.uint32)
-            got = out.download(w * h * 4, offsets[i]).view(np.uint32).reshape(h, w)
+            pitch = pitches[i]
+            raw = np.zeros(pitch * h, np.uint8)
+            raw[:pitch * (h - 1) + w * 4] = out.download(pitch * (h - 1) + w * 4, offsets[i])
+            got = np.ascontiguousarray(raw.reshape(h, pitch)[:, :w * 4]).view(np.uint32)
             ok = ok and np.array_equal(got, want)
         verified = ok
         log(f"roi verification bit-exact: {ok}")
@@ -198,10 +217,13 @@ def roi_leg(args, threads: int, stream):
         "requests": n, "decoded_bytes": decoded, "output_bytes": sum(w * h * 4 for *_, w, h in rois),
         "saturation": {"ms": round(sat.seconds * 1e3, 3), "requests_per_s": round(sat_qps, 1),
                        "decoded_GBps": round(sat.decoded_bytes / sat.seconds / 1e9, 2),
-                       "batches": int(sat.batches)},
+                       "batches": int(sat.batches),
+                       "algorithmic_GBps": round((touched + 4 * sum(w * h for *_, w, h in rois)) / sat.seconds / 1e9, 1),
+                       "touched_payload_bytes": touched},
+        "output_row_alignment_bytes": align,
         "latency": latency,
         "per_request_path": plain,
-        "server": {"streams": args.roi_streams, "max_batch": args.roi_max_batch},
+        "server": {"streams": args.roi_streams, "max_batch": args.roi_max_batch, "shape": list(shape)},
         "verified_bit_exact": verified,
     }
 
@@ -232,6 +254,7 @@ def main() -> int:
     p.add_argument("--roi-streams", type=int, default=3)
     p.add_argument("--roi-max-batch", type=int, default=1024)
     p.add_argument("--roi-plain-sample", type=int, default=300)
+    p.add_argument("--roi-shape", default="4,256,0,4,3", help="server launch shape group,block,grid,chunk,depth")
     p.add_argument("--qps", type=lambda v: [float(x) for x in v.split(",")], default=[100000.0, 400000.0],
                    help="Poisson arrival rates (requests/s) for the ROI latency runs")
     args = p.parse_args()

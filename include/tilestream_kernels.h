@@ -76,7 +76,8 @@ typedef struct {
 /* Request mode: a batch of region requests over resident archives.  Request r
  * covers tasks [first_task, next request's first_task): the tiles of columns
  * col0.. col0+cols-1 and rows row0.., row-major, of the archive whose tiles
- * start at tile_base in the resident tile table (tsk_decode_args.tasks). */
+ * start at tile_base in the resident tile table (tsk_decode_args.tasks).
+ * tsk_decode_args.request_index holds the request of every task. */
 typedef struct {
     uint64_t first_task;
     uint32_t tile_base;          /* index of the archive's tile 0 in the tile table */
@@ -85,9 +86,6 @@ typedef struct {
     uint32_t cols;               /* tiles per region row */
     uint32_t reserved;
 } tsk_request;
-
-/* Every `TSK_REQUEST_STRIDE` tasks, the request holding the first of them. */
-#define TSK_REQUEST_STRIDE 64u
 
 typedef struct {
     const tsk_decode_image *images;
@@ -106,7 +104,7 @@ typedef struct {
     uint32_t depth;              /* tiles a workgroup holds at once: 1 staged; 2, 3: 1 or 2 more in flight */
     uint32_t request_count;      /* request mode when non-zero (requests, request_index set) */
     const tsk_request *requests;
-    const uint32_t *request_index; /* task / TSK_REQUEST_STRIDE -> request */
+    const uint32_t *request_index; /* task -> request */
 } tsk_decode_args;
 
 /* Per-image descriptor of an encode job. */

@@ -160,15 +160,23 @@ __device__ inline float sample_float(std::uint32_t bits, bool u16) {
 }
 
 // Stores G consecutive 32-bit words at a 4-byte aligned address.  Outputs are
-// written once and never read back by the decoder, so a group of at most 16
-// bytes is stored non-temporally: consecutive lanes then cover whole cache
-// lines in one instruction.  Wider groups take several instructions per line
-// and keep ordinary stores, which the L2 merges into whole lines.
-template <int G> __device__ inline void store_words(gptr<std::uint32_t> dst, const std::uint32_t (&w)[G]) {
-    if constexpr (G > 4) {
+// written once and never read back by the decoder.  When the tile's output
+// rows cover whole 128-byte lines, no other tile writes those lines, and a
+// group of at most 16 bytes is stored non-temporally (consecutive lanes then
+// cover whole lines in one instruction).  Otherwise lines at the tile's edges
+// are shared with a neighbouring tile: ordinary stores let the L2 merge both
+// halves into one write-back instead of two partial ones.
+template <int G, bool NT> __device__ inline void store_words(gptr<std::uint32_t> dst, const std::uint32_t (&w)[G]) {
+    if constexpr (G > 4 || !NT) {
+        if constexpr (G >= 4) {
 #pragma unroll
-        for (int k = 0; k + 4 <= G; k += 4)
-            *reinterpret_cast<gptr<u32x4w>>(dst + k) = u32x4w{w[k], w[k + 1], w[k + 2], w[k + 3]};
+            for (int k = 0; k + 4 <= G; k += 4)
+                *reinterpret_cast<gptr<u32x4w>>(dst + k) = u32x4w{w[k], w[k + 1], w[k + 2], w[k + 3]};
+        } else if constexpr (G == 2) {
+            *reinterpret_cast<gptr<u32x2w>>(dst) = u32x2w{w[0], w[1]};
+        } else {
+            dst[0] = w[0];
+        }
     } else if constexpr (G == 4) {
         __builtin_nontemporal_store(u32x4w{w[0], w[1], w[2], w[3]}, reinterpret_cast<gptr<u32x4w>>(dst));
     } else if constexpr (G == 2) {
@@ -198,7 +206,7 @@ template <int G> __device__ inline void load_floats(gptr<const float> src, float
 // Stores G samples (or their calibrated values) for tile columns col0..col0+G-1.
 // Groups wholly inside the region store whole runs; groups on its left or
 // right edge store only the region's columns.
-template <int G, bool U16, bool CAL>
+template <int G, bool U16, bool CAL, bool NT = false>
 __device__ inline void emit(const tsk_decode_image &img, const Region &r, std::uint32_t col0, const Group<G> &g) {
     const bool inside = col0 >= r.left && col0 + G <= r.right;
     const std::int64_t ex0 = r.ex_tile + col0;
@@ -228,7 +236,7 @@ __device__ inline void emit(const tsk_decode_image &img, const Region &r, std::u
             for (int k = 0; k < G; ++k)
                 // Two roundings to nearest even, never contracted into an FMA.
                 out[k] = __float_as_uint(__fmul_rn(__fsub_rn(sample_float(g.v[k], U16), o[k]), gn[k]));
-            store_words<G>(reinterpret_cast<gptr<std::uint32_t>>(dst), out);
+            store_words<G, NT>(reinterpret_cast<gptr<std::uint32_t>>(dst), out);
         } else {
 #pragma unroll
             for (int k = 0; k < G; ++k) {
@@ -247,7 +255,7 @@ __device__ inline void emit(const tsk_decode_image &img, const Region &r, std::u
                 std::uint32_t w[G / 2 > 0 ? G / 2 : 1];
 #pragma unroll
                 for (int i = 0; i < G / 2; ++i) w[i] = __builtin_amdgcn_perm(g.v[2 * i + 1], g.v[2 * i], 0x05040100u);
-                if constexpr (G >= 2) store_words<G / 2>(reinterpret_cast<gptr<std::uint32_t>>(dst), w);
+                if constexpr (G >= 2) store_words<G / 2, NT>(reinterpret_cast<gptr<std::uint32_t>>(dst), w);
             } else {
 #pragma unroll
                 for (int k = 0; k < G; ++k) dst[k] = static_cast<std::uint16_t>(g.v[k]);
@@ -260,7 +268,7 @@ __device__ inline void emit(const tsk_decode_image &img, const Region &r, std::u
     } else {
         gptr<std::uint32_t> dst = reinterpret_cast<gptr<std::uint32_t>>(r.line) + ex0;
         if (inside) {
-            store_words<G>(dst, g.v);
+            store_words<G, NT>(dst, g.v);
         } else {
 #pragma unroll
             for (int k = 0; k < G; ++k)
@@ -363,6 +371,16 @@ __device__ inline Region region_of(const tsk_decode_image &img, const TileView &
     return r;
 }
 
+// Whether every output row of the tile starts on a 128-byte line and spans
+// whole lines, so that no other tile writes into them.
+__device__ inline bool whole_lines(const tsk_decode_image &img, const TileView &t) {
+    const std::uint32_t s = img.calibrated || img.type != TSK_UINT16 ? 4u : 2u;
+    const std::int64_t ex = static_cast<std::int64_t>(t.tx) - static_cast<std::int64_t>(img.roi_x);
+    const std::uintptr_t first = reinterpret_cast<std::uintptr_t>(img.out) + static_cast<std::uintptr_t>(ex) * s;
+    return t.left == 0 && t.right == img.tile_w && img.out_pitch % 128 == 0 && first % 128 == 0 &&
+           (img.tile_w * s) % 128 == 0;
+}
+
 // Decodes the staged tile and writes its region pixels as it goes.
 template <int G, bool U16, bool CAL, int BLOCK>
 __device__ void decode_tile(const tsk_decode_image &img, const TileView &t, const std::uint32_t *stage,
@@ -438,12 +456,20 @@ __device__ inline void decode_tile_held(const tsk_decode_image &img, const TileV
     }
 }
 
-template <int G, bool U16, bool CAL, int BLOCK, int K>
-__device__ inline void held_store(const tsk_decode_image &img, const TileView &t, const Held<G, K> &h) {
+template <int G, bool U16, bool CAL, int BLOCK, int K, bool NT>
+__device__ inline void held_store_as(const tsk_decode_image &img, const TileView &t, const Held<G, K> &h) {
     const std::uint32_t col0 = (threadIdx.x % (img.tile_w / G)) * G;
 #pragma unroll
     for (int k = 0; k < K; ++k)
-        if (h.row[k] < t.bottom) emit<G, U16, CAL>(img, region_of(img, t, h.row[k]), col0, h.v[k]);
+        if (h.row[k] < t.bottom) emit<G, U16, CAL, NT>(img, region_of(img, t, h.row[k]), col0, h.v[k]);
+}
+
+// Tiles whose output rows span whole lines store them non-temporally (where a
+// group is one store instruction).
+template <int G, bool U16, bool CAL, int BLOCK, int K>
+__device__ inline void held_store(const tsk_decode_image &img, const TileView &t, const Held<G, K> &h) {
+    if (whole_lines(img, t)) held_store_as<G, U16, CAL, BLOCK, K, true>(img, t, h);
+    else held_store_as<G, U16, CAL, BLOCK, K, false>(img, t, h);
 }
 
 // A tile whose payload is being fetched.
@@ -459,22 +485,10 @@ struct Pending {
     gptr<const u32x4> blocks;
 };
 
-// Request mode: the request holding task i, found by a binary search between
-// the sampled index entries around i (at most TSK_REQUEST_STRIDE requests, all
-// loads scalar), and the tile of its region.
+// Request mode: the request holding task i (one table load) and the tile of
+// its region.
 __device__ inline tsk_decode_task request_task(const tsk_decode_args &a, std::uint64_t i, std::uint32_t &request) {
-    const std::uint64_t slot = i / TSK_REQUEST_STRIDE;
-    std::uint32_t r = load_constant(a.request_index, slot);
-    std::uint32_t hi = (slot + 1) * TSK_REQUEST_STRIDE < a.task_count ? load_constant(a.request_index, slot + 1)
-                                                                      : a.request_count - 1;
-    // Invariant: requests[r].first_task <= i < requests[hi + 1].first_task.
-    while (r < hi) {
-        const std::uint32_t mid = r + (hi - r + 1) / 2;
-        if (load_constant(a.requests, mid).first_task <= i)
-            r = mid;
-        else
-            hi = mid - 1;
-    }
+    const std::uint32_t r = load_constant(a.request_index, i);
     const tsk_request q = load_constant(a.requests, r);
     const std::uint32_t local = static_cast<std::uint32_t>(i - q.first_task);
     const std::uint32_t dr = local / q.cols;

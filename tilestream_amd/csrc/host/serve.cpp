@@ -195,7 +195,7 @@ Layout layout_for(std::uint64_t requests, std::uint64_t tasks) {
     l.images = align_up(l.results + requests * sizeof(DeviceResult), 256);
     l.requests = align_up(l.images + requests * sizeof(tsk_decode_image), 256);
     l.index = align_up(l.requests + requests * sizeof(tsk_request), 256);
-    l.total = l.index + (tasks / TSK_REQUEST_STRIDE + 1) * sizeof(std::uint32_t);
+    l.total = l.index + tasks * sizeof(std::uint32_t);
     return l;
 }
 
@@ -362,10 +362,7 @@ Status Server::Impl::launch(Slot &slot, const RegionRequest *requests, std::uint
         q.row0 = e.row0;
         q.cols = e.cols;
         q.reserved = 0;
-        // Every TSK_REQUEST_STRIDE-th task names the request that holds it.
-        for (std::uint64_t s = (task + TSK_REQUEST_STRIDE - 1) / TSK_REQUEST_STRIDE * TSK_REQUEST_STRIDE;
-             s < task + e.tasks; s += TSK_REQUEST_STRIDE)
-            index[s / TSK_REQUEST_STRIDE] = static_cast<std::uint32_t>(k);
+        std::fill_n(index + task, e.tasks, static_cast<std::uint32_t>(k));
         task += e.tasks;
         u16 = u16 || a.info.image.type == PixelType::uint16;
         f32 = f32 || a.info.image.type == PixelType::float32;
