@@ -228,6 +228,90 @@ def roi_leg(args, threads: int, stream):
     }
 
 
+def fp64_errors(got_bits: np.ndarray, x: np.ndarray, offset: np.ndarray, gain: np.ndarray):
+    """ULP error of FP32 (x - offset) * gain against the FP64 value, over
+    finite normal references: max / p99 / median, and results beyond the
+    1.5 ulp bound of two roundings; non-finite results must match by class."""
+    xf = x.astype(np.float64) if x.dtype == np.uint16 else x.view(np.float32).astype(np.float64)
+    ref = (xf - offset.astype(np.float64)) * gain.astype(np.float64)
+    got = got_bits.view(np.float32).astype(np.float64)
+    finite = np.isfinite(ref)
+    with np.errstate(invalid="ignore"):
+        ref32 = ref.astype(np.float32)
+    normal = finite & (np.abs(ref32) >= np.finfo(np.float32).tiny) & np.isfinite(ref32)
+    ulp = np.spacing(np.abs(ref32[normal])).astype(np.float64)
+    err = np.abs(got[normal] - ref[normal]) / ulp
+    nan_mismatch = int(np.sum(np.isnan(ref) != np.isnan(got)))
+    inf_mismatch = int(np.sum(np.isinf(ref) & (got != ref)))
+    return {"max_ulp": round(float(err.max()), 4), "p99_ulp": round(float(np.percentile(err, 99)), 4),
+            "median_ulp": round(float(np.median(err)), 4), "violations_over_1_5_ulp": int(np.sum(err > 1.5)),
+            "nonfinite_class_mismatches": nan_mismatch + inf_mismatch, "pixels": int(x.size)}
+
+
+def fused_leg(args, stream):
+    """Plain method at real shape: one 4096x4096 uint16 detector frame and one
+    4096x4096 float32 field (64x64 tiles), whole decode fused with a per-pixel
+    dark map (uint16 U[90,110], exact in FP32) and an FP32 gain map N(1, 0.02).
+    One plan per image; four buffer sets rotate so the 256 MB last-level
+    cache cannot hold a set between steps."""
+    from tilestream_amd import api, generators
+    side, tile, sets = 4096, (64, 64), 4
+    out = {}
+    for kind in ("u16", "f32"):
+        px = generators.image_for(side, side, np.uint16 if kind == "u16" else np.uint32, args.seed + 11)
+        st, archive = api.cpu_compress(px, api.TileShape(*tile))
+        assert st == api.Status.success, st
+        st, meta = api.parse_metadata(archive)
+        dark = generators.dark_u16(side, side, args.seed + 12).astype(np.float32)
+        gain = generators.gain_f32(side, side, args.seed + 13)
+        keep, plans = [], []
+        ws_bytes = 0
+        for k in range(sets):
+            src = api.DeviceMemory.from_host(np.frombuffer(archive, np.uint8))
+            om, gm = api.DeviceMemory.from_host(dark), api.DeviceMemory.from_host(gain)
+            dst = api.DeviceMemory(side * side * 4)
+            it = api.BatchItem()
+            it.compressed_input.bytes = api.ConstBytes(src.ptr, len(archive))
+            it.compressed_input.metadata = meta.handle
+            it.roi = api.Roi(0, 0, side, side)
+            it.output = api.device_view(dst, api.PixelType.float32, side, side)
+            it.calibration.offset = api.map_parameter(api.device_view(om, api.PixelType.float32, side, side, const=True))
+            it.calibration.gain = api.map_parameter(api.device_view(gm, api.PixelType.float32, side, side, const=True))
+            st, plan = api.prepare_batch(api.Operation.decode_calibrate, [it], stream=stream)
+            assert st == api.Status.success, st
+            ws_bytes = max(ws_bytes, plan.workspace_bytes())
+            plans.append(plan)
+            keep.append((src, om, gm, dst, it))
+        ws = api.DeviceMemory(max(256, ws_bytes))
+        res = api.DeviceMemory(16)
+        step = [0]
+
+        def run():
+            api.decode_calibrate_async(plans[step[0] % sets], ws, res, 1, stream)
+            step[0] += 1
+
+        for _ in range(args.warmup):
+            run()
+        ms = timed(stream, run, args.steps)
+        stream.synchronize()
+        status, written = api.read_results(res, 1)
+        got = keep[(step[0] - 1) % sets][3].download().view(np.uint32).reshape(side, side)
+        payload = sum(meta.tile(k).length for k in range(meta.summary.tile_count))
+        alg = payload + side * side * (4 + 8)
+        out[kind] = {
+            "ms": round(ms, 4), "decoded_GBps": round(px.nbytes / (ms * 1e-3) / 1e9, 1),
+            "algorithmic_GBps": round(alg / (ms * 1e-3) / 1e9, 1), "roofline_frac": round(alg / (ms * 1e-3) / 1e9 / HBM_PEAK_GBPS, 4),
+            "algorithmic_bytes": alg, "status": int(status[0]),
+            "fp64_error": fp64_errors(got, px, dark, gain),
+        }
+        log(f"fused {kind}: {ms:.4f} ms, {out[kind]['algorithmic_GBps']} GB/s algorithmic, "
+            f"max {out[kind]['fp64_error']['max_ulp']} ulp")
+        del keep, plans
+    return {"workload": "4096x4096 uint16 detector frame and 4096x4096 float32 field, 64x64 tiles, whole "
+                        "decode fused with dark map U[90,110] and gain map N(1,0.02), one plan per image, "
+                        "4 rotating buffer sets", **out}
+
+
 def main() -> int:
     p = argparse.ArgumentParser()
     p.add_argument("--gpus", type=int, default=1)
@@ -248,6 +332,7 @@ def main() -> int:
                    help="output rows start at multiples of this many bytes, as hipMallocPitch "
                         "allocates them (1: packed rows)")
     p.add_argument("--no-roi", action="store_true", help="skip the ROI serving leg")
+    p.add_argument("--no-fused", action="store_true", help="skip the fused-calibration real-shape leg")
     p.add_argument("--roi-only", action="store_true", help="only the ROI serving leg")
     p.add_argument("--roi-archives", type=int, default=64)
     p.add_argument("--roi-requests", type=int, default=10000)
@@ -384,6 +469,7 @@ def main() -> int:
     cpu = None if args.no_cpu else cpu_baseline(batch, threads, 1_000_000_000)
     del plans, pws, rec
     roi = None if args.no_roi else roi_leg(args, threads, stream)
+    fused = None if args.no_fused else fused_leg(args, stream)
     if cpu:
         log(f"cpu baseline {cpu['value']} GB/s on {cpu['cores']} threads ({cpu['kind']})")
 
@@ -428,6 +514,7 @@ def main() -> int:
         "cpu_baseline": cpu,
         "verified_bit_exact": verified,
         "roi": roi,
+        "fused_real_shape": fused,
     }
     if roi is not None:
         out["p99_roi_latency_us"] = {str(int(r["qps"])): r["p99_us"] for r in roi["latency"]}
