@@ -83,6 +83,21 @@ def cpu_baseline(batch, threads: int, single_budget_bytes: int):
     }
 
 
+def profiled_traffic():
+    """HBM bytes per launch of this workload, measured with rocprofv3 FETCH_SIZE
+    and WRITE_SIZE counters (scaled by a copy kernel of known size) and recorded
+    in the committed profile summary; None when the summary is absent."""
+    path = os.path.join(ROOT, "profiles", "decode_batch256_kernel_trace.md")
+    try:
+        with open(path) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    return int(line.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
+
+
 def percentile(values, q: float) -> float:
     """Nearest-rank percentile (the reference harness's convention)."""
     v = np.sort(np.asarray(values))
@@ -477,6 +492,8 @@ def main() -> int:
     gbps = decoded / (ms_batched * 1e-3) / 1e9
     gbps_plain = decoded / (ms_plain * 1e-3) / 1e9
     traffic_alg = stats.payload_bytes + stats.output_bytes + stats.map_bytes
+    default_workload = args.images == 256 and args.seed == 2026517 and args.pitch_align == 256
+    traffic = profiled_traffic() if default_workload else None
     achieved = traffic_alg / (ms_batched * 1e-3) / 1e9
     out = {
         "metric": METRIC,
@@ -504,10 +521,12 @@ def main() -> int:
         "speedup_vs_per_image": round(ms_plain / ms_batched, 3),
         "roofline": {
             "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": None,
+            "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": traffic,
             "algorithmic_bytes_per_launch": traffic_alg,
             "note": "payload bytes of every tile read + decoded output bytes written, over the "
-                    "graph launch time (results init, counter reset, decode, results completion)",
+                    "graph launch time (results init, counter reset, decode, results completion; the "
+                    "decode kernel is 98-99% of it); traffic = HBM bytes per launch from rocprofv3 "
+                    "FETCH_SIZE/WRITE_SIZE on this workload (profiles/decode_batch256_kernel_trace.md)",
         },
         "autotune": {"group": best.group, "block": best.block, "chunk": best.chunk, "depth": best.depth,
                      "candidates": len(candidates)},
