@@ -158,7 +158,7 @@ def roi_leg(args, threads: int, stream):
     # Saturation: every request arrives at once; continuous batching drains them.
     sat = None
     for _ in range(2):
-        st, lat, statuses, stats = server.replay(requests, np.zeros(n, np.uint64))
+        st, lat, statuses, _, stats = server.replay(requests, np.zeros(n, np.uint64))
         assert st == api.Status.success and (statuses == 0).all(), st
         sat = stats
     sat_qps = n / sat.seconds
@@ -168,7 +168,7 @@ def roi_leg(args, threads: int, stream):
     latency = []
     for q in args.qps:
         arrivals = np.cumsum(rng.exponential(1e9 / q, n)).astype(np.uint64)
-        st, lat, statuses, stats = server.replay(requests, arrivals)
+        st, lat, statuses, _, stats = server.replay(requests, arrivals)
         assert st == api.Status.success and (statuses == 0).all(), st
         rec = {"qps": q, "achieved_qps": round(n / (arrivals[-1] * 1e-9), 1),
                "p50_us": round(percentile(lat, 50) / 1e3, 1), "p99_us": round(percentile(lat, 99) / 1e3, 1),
@@ -211,7 +211,7 @@ def roi_leg(args, threads: int, stream):
         # Re-run the saturation replay into cleared outputs, then check a sample
         # against host binary32 (x - 100) * 1.05 of the generated pixels.
         out.fill(0)
-        st, _, statuses, _ = server.replay(requests, np.zeros(n, np.uint64))
+        st, _, statuses, _, _ = server.replay(requests, np.zeros(n, np.uint64))
         ok = st == api.Status.success and bool((statuses == 0).all())
         big = max(range(n), key=lambda i: rois[i][3] * rois[i][4])
         o32, g32 = np.float32(offset), np.float32(gain)
@@ -238,6 +238,152 @@ def roi_leg(args, threads: int, stream):
         "output_row_alignment_bytes": align,
         "latency": latency,
         "per_request_path": plain,
+        "server": {"streams": args.roi_streams, "max_batch": args.roi_max_batch, "shape": list(shape)},
+        "verified_bit_exact": verified,
+    }
+
+
+def serve_mix_leg(args, threads: int, stream):
+    """Serving mix: Poisson arrivals of region decodes (70%: sides log-uniform in
+    [32, 2048] clipped to the image, uniform origins), whole-image decodes (20%)
+    and GPU encodes (10%) over images whose width and height are drawn from
+    {512, 1024, 2048, 4096, 8192}, 50/50 uint16 / float32, tiles from {128x32,
+    64x64, 256x16}; archives and encode inputs resident in HBM; 200 GB
+    admission cap on the output bytes in flight."""
+    from tilestream_amd import api
+    from tilestream_amd.workloads import ResidentBatch, _align
+
+    rng = np.random.default_rng(args.seed + 20)
+    sides = (512, 1024, 2048, 4096, 8192)
+    tiles = ((128, 32), (64, 64), (256, 16))
+    t0 = time.perf_counter()
+    layout = [(int(rng.choice(sides)), int(rng.choice(sides)), np.uint16 if k % 2 == 0 else np.uint32,
+               tiles[int(rng.integers(0, len(tiles)))]) for k in range(args.mix_images)]
+    rb = ResidentBatch.build(layout, args.seed + 21, threads)
+    aset = api.ArchiveSet()
+    for e in rb.entries:
+        st, _ = aset.add(rb.src, e.meta, len(e.archive), e.src_offset)
+        assert st == api.Status.success, st
+    assert aset.commit(stream) == api.Status.success
+    raw_offsets, total = [], 0
+    for e in rb.entries:
+        raw_offsets.append(total)
+        total += _align(e.raw_bytes)
+    raw = api.DeviceMemory(total)
+    for e, off in zip(rb.entries, raw_offsets):
+        raw.upload(e.pixels, off)
+
+    n = args.mix_requests
+    kinds = rng.choice(3, size=n, p=[0.7, 0.2, 0.1])  # 0 region, 1 whole, 2 encode
+    images = rng.integers(0, len(rb.entries), n)
+    lo, hi = np.log(32.0), np.log(2048.0)
+    specs, out_total, enc_total = [], 0, 0
+    for k, a in zip(kinds, images):
+        e = rb.entries[int(a)]
+        if k == 2:
+            st, bound = api.compress_bound(api.ImageSpec(int(e.ptype), e.width, e.height), api.TileShape(*e.tile))
+            assert st == api.Status.success
+            specs.append((2, int(a), None, enc_total, bound, 0))
+            enc_total += _align(bound)
+            continue
+        if k == 1:
+            x, y, w, h = 0, 0, e.width, e.height
+        else:
+            w = min(e.width, int(np.exp(rng.uniform(lo, hi))))
+            h = min(e.height, int(np.exp(rng.uniform(lo, hi))))
+            x, y = int(rng.integers(0, e.width - w + 1)), int(rng.integers(0, e.height - h + 1))
+        pitch = (w * e.sample + args.pitch_align - 1) // args.pitch_align * args.pitch_align
+        span = pitch * (h - 1) + w * e.sample
+        specs.append((int(k), int(a), (x, y, w, h), out_total, span, pitch))
+        out_total += _align(span)
+    out = api.DeviceMemory(max(256, out_total))
+    enc = api.DeviceMemory(max(256, enc_total))
+    requests = []
+    for k, a, roi, off, size, pitch in specs:
+        e = rb.entries[a]
+        if k == 2:
+            view = api.device_view(raw, e.ptype, e.width, e.height, offset=raw_offsets[a], capacity=e.raw_bytes,
+                                   const=True)
+            requests.append(api.encode_request(view, e.tile, api.MutableBytes(enc.ptr + off, size)))
+        else:
+            view = api.device_view(out, e.ptype, roi[2], roi[3], pitch, offset=off, capacity=size)
+            requests.append(api.region_request(a, roi, view))
+    counts = [int(np.sum(kinds == k)) for k in range(3)]
+    log(f"serve mix: {len(rb.entries)} images ({rb.decoded_bytes / 1e9:.2f} GB raw, {rb.archive_bytes / 1e9:.2f} GB "
+        f"archives), {n} requests ({counts[0]} region, {counts[1]} whole, {counts[2]} encode), "
+        f"{out_total / 1e9:.1f} GB decode outputs, {enc_total / 1e9:.1f} GB encode outputs, "
+        f"set up in {time.perf_counter() - t0:.1f} s")
+
+    shape = tuple(int(v) for v in args.roi_shape.split(","))
+    server = api.Server(aset, api.server_options(streams=args.roi_streams, max_batch=args.roi_max_batch,
+                                                 admission_bytes=200 * 10**9, shape=shape))
+    sat = None
+    for _ in range(2):
+        st, lat, statuses, written, stats = server.replay(requests, np.zeros(n, np.uint64))
+        assert st == api.Status.success and (statuses == 0).all(), st
+        sat = stats
+    log(f"serve mix saturation: {sat.seconds * 1e3:.2f} ms, {n / sat.seconds:.0f} requests/s, "
+        f"{sat.decoded_bytes / sat.seconds / 1e9:.1f} GB/s, {sat.batches} batches")
+    arrivals_rng = np.random.default_rng(args.seed + 22)
+    latency = []
+    for q in args.mix_qps:
+        arrivals = np.cumsum(arrivals_rng.exponential(1e9 / q, n)).astype(np.uint64)
+        st, lat, statuses, _, stats = server.replay(requests, arrivals)
+        assert st == api.Status.success and (statuses == 0).all(), st
+        rec = {"qps": q, "achieved_qps": round(n / (arrivals[-1] * 1e-9), 1),
+               "p50_us": round(percentile(lat, 50) / 1e3, 1), "p99_us": round(percentile(lat, 99) / 1e3, 1),
+               "batches": int(stats.batches), "GBps": round(stats.decoded_bytes / stats.seconds / 1e9, 1)}
+        for k, name in enumerate(("region", "whole", "encode")):
+            sel = lat[kinds == k]
+            if sel.size:
+                rec[name] = {"p50_us": round(percentile(sel, 50) / 1e3, 1),
+                             "p99_us": round(percentile(sel, 99) / 1e3, 1)}
+        log(f"serve mix at {q} req/s: p50 {rec['p50_us']} us, p99 {rec['p99_us']} us; region p99 "
+            f"{rec['region']['p99_us']} us, whole p99 {rec['whole']['p99_us']} us, "
+            f"encode p99 {rec['encode']['p99_us']} us")
+        latency.append(rec)
+
+    verified = None
+    if not args.no_verify:
+        # Replay into cleared outputs; check every encode's size, a sample of
+        # encodes byte for byte against the host codec's archive, and a sample
+        # of decodes against the generated pixels.
+        out.fill(0)
+        enc.fill(0)
+        st, _, statuses, written, _ = server.replay(requests, np.zeros(n, np.uint64))
+        ok = st == api.Status.success and bool((statuses == 0).all())
+        checked = {0: 0, 1: 0, 2: 0}
+        limit = {0: 200, 1: 12, 2: 12}
+        for i, (k, a, roi, off, size, pitch) in enumerate(specs):
+            e = rb.entries[a]
+            if k == 2:
+                ok = ok and int(written[i]) == len(e.archive)
+                if checked[2] < limit[2]:
+                    ok = ok and enc.download(len(e.archive), off).tobytes() == e.archive
+                    checked[2] += 1
+                continue
+            if checked[k] >= limit[k]:
+                continue
+            x, y, w, h = roi
+            buf = np.zeros(pitch * h, np.uint8)
+            buf[:size] = out.download(size, off)
+            got = np.ascontiguousarray(buf.reshape(h, pitch)[:, :w * e.sample]).view(e.pixels.dtype)
+            ok = ok and np.array_equal(got, e.pixels[y:y + h, x:x + w])
+            checked[k] += 1
+        verified = bool(ok)
+        log(f"serve mix verification bit-exact: {verified} (checked {checked})")
+    return {
+        "workload": f"{n} requests (70% region decode, sides log-uniform in [32, 2048]; 20% whole decode; "
+                    f"10% GPU encode) over {len(rb.entries)} images with sides from {{512..8192}}, 50/50 "
+                    f"uint16/float32, tiles from {{128x32, 64x64, 256x16}}; Poisson arrivals; admission cap "
+                    f"200 GB of output bytes in flight",
+        "requests": {"region": counts[0], "whole": counts[1], "encode": counts[2]},
+        "images_raw_bytes": rb.decoded_bytes,
+        "saturation": {"ms": round(sat.seconds * 1e3, 3), "requests_per_s": round(n / sat.seconds, 1),
+                       "GBps": round(sat.decoded_bytes / sat.seconds / 1e9, 2), "batches": int(sat.batches),
+                       "bytes": int(sat.decoded_bytes),
+                       "note": "bytes = pixels decoded (stored sample size) + pixels encoded"},
+        "latency": latency,
         "server": {"streams": args.roi_streams, "max_batch": args.roi_max_batch, "shape": list(shape)},
         "verified_bit_exact": verified,
     }
@@ -357,6 +503,12 @@ def main() -> int:
     p.add_argument("--roi-shape", default="4,256,0,4,3", help="server launch shape group,block,grid,chunk,depth")
     p.add_argument("--qps", type=lambda v: [float(x) for x in v.split(",")], default=[100000.0, 400000.0],
                    help="Poisson arrival rates (requests/s) for the ROI latency runs")
+    p.add_argument("--no-mix", action="store_true", help="skip the serving-mix leg")
+    p.add_argument("--mix-only", action="store_true", help="only the serving-mix leg")
+    p.add_argument("--mix-images", type=int, default=24)
+    p.add_argument("--mix-requests", type=int, default=3000)
+    p.add_argument("--mix-qps", type=lambda v: [float(x) for x in v.split(",")], default=[5000.0, 20000.0],
+                   help="Poisson arrival rates (requests/s) for the serving-mix latency runs")
     args = p.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (this codec has no "
@@ -380,6 +532,10 @@ def main() -> int:
         log(f"copy probe: {args.steps} copies of {nbytes} bytes (read {nbytes}, write {nbytes} each)")
         return 0
     threads = max(1, min(16, os.cpu_count() or 1))
+    if args.mix_only:
+        mix = serve_mix_leg(args, threads, api.Stream())
+        print(json.dumps({"metric": "serving mix throughput and latency", "serve_mix": mix}), flush=True)
+        return 0 if mix["verified_bit_exact"] in (None, True) else 3
     if args.roi_only:
         roi = roi_leg(args, threads, api.Stream())
         print(json.dumps({"metric": "p99 ROI latency", "roi": roi}), flush=True)
@@ -485,6 +641,7 @@ def main() -> int:
     del plans, pws, rec
     roi = None if args.no_roi else roi_leg(args, threads, stream)
     fused = None if args.no_fused else fused_leg(args, stream)
+    mix = None if args.no_mix else serve_mix_leg(args, threads, stream)
     if cpu:
         log(f"cpu baseline {cpu['value']} GB/s on {cpu['cores']} threads ({cpu['kind']})")
 
@@ -534,11 +691,13 @@ def main() -> int:
         "verified_bit_exact": verified,
         "roi": roi,
         "fused_real_shape": fused,
+        "serve_mix": mix,
     }
     if roi is not None:
         out["p99_roi_latency_us"] = {str(int(r["qps"])): r["p99_us"] for r in roi["latency"]}
     print(json.dumps(out), flush=True)
-    ok = verified in (None, True) and (roi is None or roi["verified_bit_exact"] in (None, True))
+    ok = verified in (None, True) and all(leg is None or leg["verified_bit_exact"] in (None, True)
+                                          for leg in (roi, mix))
     return 0 if ok else 3
 
 

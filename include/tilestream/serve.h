@@ -1,5 +1,6 @@
 // Serving resident archives: an extension of the reference API (no counterpart
 // in the reference, which plans every request on the host with prepare_batch).
+// Requests decode regions of resident archives or encode device images.
 //
 // ArchiveSet keeps archives resident in device memory together with their tile
 // and row tables, planned once when an archive is added.  A batch of region
@@ -51,14 +52,26 @@ class ArchiveSet {
     std::unique_ptr<Impl> impl_;
 };
 
-// A region of a resident archive, decoded bit-exact (calibrated = false) or
-// fused with scalar calibration into FP32.
-struct RegionRequest {
-    std::uint32_t archive = 0;
-    std::uint32_t calibrated = 0;
-    Roi roi;
-    ImageView output; // device; archive pixel type, or float32 when calibrated
-    float offset = 0.0F;
+enum class RequestKind : std::uint32_t {
+    region = 0, // decode a region (or all) of a resident archive
+    encode = 1, // compress a device image into a canonical .tsc file
+};
+
+// One request.  region: a region of a resident archive, decoded bit-exact
+// (calibrated = 0) or fused with scalar calibration into FP32 (calibrated = 1).
+// encode: the device image `input` compressed with tile shape `tile` into
+// `compressed` (device, at least compress_bound bytes); the result's
+// bytes_written is the file size.
+struct ServeRequest {
+    RequestKind kind = RequestKind::region;
+    std::uint32_t archive = 0;    // region
+    std::uint32_t calibrated = 0; // region
+    TileShape tile;               // encode
+    Roi roi;                      // region
+    ImageView output;             // region: device; archive pixel type, or float32 when calibrated
+    ConstImageView input;         // encode: device pixels
+    MutableBytes compressed;      // encode: device output
+    float offset = 0.0F;          // region, calibrated
     float gain = 1.0F;
 };
 
@@ -66,14 +79,14 @@ struct ServerOptions {
     std::uint32_t streams = 3;           // batches in flight at once
     std::uint32_t max_batch = 1024;      // requests per batch
     std::uint32_t max_tasks = 1u << 22;  // tiles per batch (a larger request is rejected)
-    std::uint64_t admission_bytes = 200ull << 30; // output bytes in flight
+    std::uint64_t admission_bytes = 200ull << 30; // output bytes in flight (encode: its compress bound)
     LaunchShape shape;                   // decode kernel launch shape (grid 0: sized per batch)
 };
 
 struct ServerStats {
     std::uint64_t requests = 0;
     std::uint64_t batches = 0;
-    std::uint64_t decoded_bytes = 0;     // region pixels x stored sample bytes
+    std::uint64_t decoded_bytes = 0;     // region pixels x stored sample bytes, plus encoded input bytes
     std::uint64_t failed = 0;            // requests whose device result was not success
     double seconds = 0.0;                // first arrival to last completion
 };
@@ -88,20 +101,22 @@ class Server {
                                        std::unique_ptr<Server> &server) noexcept;
 
     // Validates a request against the archive set (no device work).
-    [[nodiscard]] Status check(const RegionRequest &request) const noexcept;
+    [[nodiscard]] Status check(const ServeRequest &request) const noexcept;
 
     // Open-loop replay: request i arrives arrival_ns[i] nanoseconds after the
     // start (non-decreasing).  latency_ns[i] (optional) receives completion -
-    // arrival as seen by the host, statuses[i] (optional) the request's device
-    // status.  Every request is validated before the replay starts; outputs must
-    // be pairwise disjoint and disjoint from the archives.
-    [[nodiscard]] Status replay(const RegionRequest *requests, const std::uint64_t *arrival_ns,
-                                std::uint64_t count, std::uint64_t *latency_ns, std::uint32_t *statuses,
+    // arrival as seen by the host, results[i] (optional, host memory) the
+    // request's device result (status; bytes_written: output bytes, the file
+    // size of an encode).  Every request is validated before the replay starts;
+    // outputs must be pairwise disjoint and disjoint from the archives and the
+    // encode inputs.
+    [[nodiscard]] Status replay(const ServeRequest *requests, const std::uint64_t *arrival_ns,
+                                std::uint64_t count, std::uint64_t *latency_ns, DeviceResult *results,
                                 ServerStats &stats) noexcept;
 
-    // Decodes `count` requests, all arriving at once, and waits for them.
-    [[nodiscard]] Status run_batch(const RegionRequest *requests, std::uint64_t count,
-                                   std::uint32_t *statuses) noexcept;
+    // Runs `count` requests, all arriving at once, and waits for them.
+    [[nodiscard]] Status run_batch(const ServeRequest *requests, std::uint64_t count,
+                                   DeviceResult *results) noexcept;
 
     struct Impl;
 

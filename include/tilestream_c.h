@@ -149,17 +149,21 @@ uint32_t ts_queue_autotune(ts_queue *queue, ts_mutable_bytes workspace, ts_devic
 
 /* ---- extension: serving resident archives (tilestream/serve.h) ---------
  * No counterpart in the reference.  Archives stay in device memory with tile
- * tables planned once; region requests are batched continuously over several
- * streams. */
+ * tables planned once; region decodes and encodes are batched continuously
+ * over several streams (hip::ServeRequest). */
 typedef struct ts_archive_set ts_archive_set;
 typedef struct ts_server ts_server;
 typedef struct { ts_image_spec image; ts_tile_shape tile; uint64_t tile_count, file_bytes; } ts_archive_info;
 typedef struct {
-    uint32_t archive, calibrated;  /* calibrated: FP32 (x - offset) * gain output */
-    ts_roi roi;
-    ts_image_view output;
+    uint32_t kind;                 /* 0 region decode, 1 encode */
+    uint32_t archive, calibrated;  /* region; calibrated: FP32 (x - offset) * gain output */
+    ts_tile_shape tile;            /* encode */
+    ts_roi roi;                    /* region */
+    ts_image_view output;          /* region: device pixels */
+    ts_const_image_view input;     /* encode: device pixels */
+    ts_mutable_bytes compressed;   /* encode: device, at least compress_bound bytes */
     float offset, gain;
-} ts_region_request;
+} ts_serve_request;
 typedef struct {
     uint32_t streams, max_batch, max_tasks, reserved;
     uint64_t admission_bytes;
@@ -178,12 +182,13 @@ void ts_server_default_options(ts_server_options *out);
 /* The archive set must outlive the server. */
 uint32_t ts_server_create(const ts_archive_set *set, const ts_server_options *options, ts_server **server);
 void ts_server_destroy(ts_server *server);
-uint32_t ts_server_check(const ts_server *server, const ts_region_request *request);
-/* Open-loop replay; arrival_ns non-decreasing; latency_ns and statuses may be NULL. */
-uint32_t ts_server_replay(ts_server *server, const ts_region_request *requests, const uint64_t *arrival_ns,
-                          uint64_t count, uint64_t *latency_ns, uint32_t *statuses, ts_server_stats *stats);
-uint32_t ts_server_run_batch(ts_server *server, const ts_region_request *requests, uint64_t count,
-                             uint32_t *statuses);
+uint32_t ts_server_check(const ts_server *server, const ts_serve_request *request);
+/* Open-loop replay; arrival_ns non-decreasing; latency_ns and results (host
+ * memory: status, and bytes_written, the file size of an encode) may be NULL. */
+uint32_t ts_server_replay(ts_server *server, const ts_serve_request *requests, const uint64_t *arrival_ns,
+                          uint64_t count, uint64_t *latency_ns, ts_device_result *results, ts_server_stats *stats);
+uint32_t ts_server_run_batch(ts_server *server, const ts_serve_request *requests, uint64_t count,
+                             ts_device_result *results);
 
 /* ---- HIP runtime helpers for hosts without their own HIP binding --------
  * These return hipError_t values (0 = hipSuccess). */

@@ -1,6 +1,7 @@
 """Serving resident archives: region requests decoded in request mode (tiles
-derived on the device from each request's region), continuous batching over
-several streams, each result identical alone and in the mix, request checks."""
+derived on the device from each request's region) and encode requests in the
+same batches, continuous batching over several streams, each result identical
+alone and in the mix, request checks."""
 import numpy as np
 import pytest
 
@@ -99,7 +100,7 @@ def test_batch_matches_oracle(orc, resident, shape):
     specs = _requests(resident, rng, 90)
     out = Outputs(resident, specs)
     server = api.Server(resident.set, api.server_options(streams=3, max_batch=16, shape=shape))
-    st, statuses = server.run_batch(out.requests)
+    st, statuses, _ = server.run_batch(out.requests)
     assert st == ts.Status.success
     assert (statuses == 0).all()
     for i, spec in enumerate(specs):
@@ -121,7 +122,7 @@ def test_many_single_tile_requests_in_one_batch(orc, resident):
     specs.append((2, (0, 0, 1000, 600), False))  # 4 x 38 = 152 tiles after them
     out = Outputs(resident, specs)
     server = api.Server(resident.set, api.server_options(streams=2, max_batch=1024))
-    st, statuses = server.run_batch(out.requests)
+    st, statuses, _ = server.run_batch(out.requests)
     assert st == ts.Status.success and (statuses == 0).all()
     for i, spec in enumerate(specs):
         assert _same(out.read(specs, i), _expected(orc, resident, spec)), (i, spec)
@@ -132,11 +133,11 @@ def test_alone_and_in_the_mix_are_identical(resident):
     specs = _requests(resident, rng, 60)
     mix = Outputs(resident, specs)
     server = api.Server(resident.set, api.server_options(streams=3, max_batch=8))
-    st, statuses = server.run_batch(mix.requests)
+    st, statuses, _ = server.run_batch(mix.requests)
     assert st == ts.Status.success and (statuses == 0).all()
     for i in range(0, len(specs), 4):
         alone = Outputs(resident, [specs[i]])
-        st, statuses = server.run_batch(alone.requests)
+        st, statuses, _ = server.run_batch(alone.requests)
         assert st == ts.Status.success and statuses[0] == 0
         assert np.array_equal(alone.read([specs[i]], 0), mix.read(specs, i)), (i, specs[i])
 
@@ -148,7 +149,7 @@ def test_replay_with_poisson_arrivals(orc, resident):
     arrivals = np.cumsum(rng.exponential(20_000.0, len(specs))).astype(np.uint64)  # 50k requests/s
     server = api.Server(resident.set, api.server_options(streams=3, max_batch=32,
                                                          admission_bytes=4 << 20))
-    st, latency, statuses, stats = server.replay(out.requests, arrivals)
+    st, latency, statuses, _, stats = server.replay(out.requests, arrivals)
     assert st == ts.Status.success
     assert (statuses == 0).all() and stats.failed == 0
     assert stats.requests == len(specs) and 1 <= stats.batches <= len(specs)
@@ -164,7 +165,7 @@ def test_admission_smaller_than_one_request_still_progresses(orc, resident):
     specs = _requests(resident, rng, 20, big_every=2)
     out = Outputs(resident, specs)
     server = api.Server(resident.set, api.server_options(streams=2, admission_bytes=1024))
-    st, latency, statuses, stats = server.replay(out.requests, np.zeros(len(specs), np.uint64))
+    st, latency, statuses, _, stats = server.replay(out.requests, np.zeros(len(specs), np.uint64))
     assert st == ts.Status.success and (statuses == 0).all()
     assert stats.batches >= len(specs) // 2  # the large regions go one at a time
     for i, spec in enumerate(specs):
@@ -193,12 +194,12 @@ def test_request_checks(resident):
     # outputs of one replay must not overlap
     a = api.region_request(0, (0, 0, 64, 64), api.device_view(mem, ts.PixelType.uint16, 64, 64))
     b = api.region_request(0, (64, 0, 64, 64), api.device_view(mem, ts.PixelType.uint16, 64, 64, offset=4096))
-    st, _ = server.run_batch([a, b])
+    st, *_ = server.run_batch([a, b])
     assert st == ts.Status.invalid_argument
     # nor write into an archive
     arch = resident.mems[0]
     c = api.region_request(0, (0, 0, 8, 8), api.device_view(arch, ts.PixelType.uint16, 8, 8, offset=128))
-    st, _ = server.run_batch([c])
+    st, *_ = server.run_batch([c])
     assert st == ts.Status.invalid_argument
     # arrivals must not decrease
     d = api.region_request(0, (64, 0, 64, 64), api.device_view(mem, ts.PixelType.uint16, 64, 64, offset=16384))
@@ -223,3 +224,109 @@ def test_archive_set_rejects_bad_archives(orc):
     assert len(s) == 0
     with pytest.raises(RuntimeError):
         api.Server(s)  # nothing committed
+
+
+ENCODES = [("u16", 300, 200, (128, 32)), ("f32", 129, 77, (64, 64)), ("u16", 1, 1, (256, 16)),
+           ("f32", 700, 90, (256, 16)), ("u16", 640, 32 * 900, (128, 32)), ("u16", 64, 64, (64, 64))]
+
+
+class EncodeJobs:
+    """Device inputs (rows padded past the width) and bound-sized outputs with guard bytes."""
+
+    def __init__(self, specs, seed, guard=64):
+        self.pixels, self.inputs, self.outs, self.bounds, self.requests = [], [], [], [], []
+        for i, (kind, w, h, tile) in enumerate(specs):
+            px = g.detector_u16(w, h, seed + i) if kind == "u16" else g.field_f32(w, h, seed + i)
+            host = np.zeros((h, w + 3), px.dtype)
+            host[:, :w] = px
+            src = api.DeviceMemory.from_host(host)
+            view = api.device_view(src, api.pixel_type_of(px.dtype), w, h, (w + 3) * px.itemsize, const=True)
+            st, bound = api.compress_bound(view.image, api.TileShape(*tile))
+            assert st == ts.Status.success
+            out = api.DeviceMemory(bound + guard)
+            out.fill(0xA5)
+            self.pixels.append(px)
+            self.inputs.append(src)
+            self.outs.append(out)
+            self.bounds.append(bound)
+            self.requests.append(api.encode_request(view, tile, api.MutableBytes(out.ptr, bound)))
+
+    def file(self, i, written):
+        raw = self.outs[i].download()
+        assert (raw[self.bounds[i]:] == 0xA5).all(), "guard bytes after the bound were written"
+        return raw[:written].tobytes()
+
+
+def test_encode_requests_match_oracle_in_a_mixed_batch(orc, resident):
+    """Encodes (both pixel types, one image over 4096 tiles) interleaved with regions."""
+    rng = np.random.default_rng(404)
+    specs = _requests(resident, rng, 30)
+    out = Outputs(resident, specs)
+    jobs = EncodeJobs(ENCODES, 410)
+    requests = list(out.requests)
+    where = []
+    for k, r in enumerate(jobs.requests):
+        at = 1 + 5 * k
+        requests.insert(at, r)
+        where.append(at)
+    server = api.Server(resident.set, api.server_options(streams=2, max_batch=64))
+    st, statuses, written = server.run_batch(requests)
+    assert st == ts.Status.success and (statuses == 0).all()
+    for k, at in enumerate(where):
+        want = orc.compress(jobs.pixels[k], ENCODES[k][3])
+        assert written[at] == len(want), (k, ENCODES[k])
+        assert jobs.file(k, int(written[at])) == want, (k, ENCODES[k])
+    regions = [i for i in range(len(requests)) if i not in where]
+    for i, spec in enumerate(specs):
+        assert _same(out.read(specs, i), _expected(orc, resident, spec)), (i, spec)
+        assert written[regions[i]] == spec[1][2] * spec[1][3] * (4 if spec[2] else resident.pixels[spec[0]].itemsize)
+
+
+def test_encode_alone_and_in_the_mix_are_identical(resident):
+    rng = np.random.default_rng(405)
+    specs = _requests(resident, rng, 24)
+    out = Outputs(resident, specs)
+    jobs = EncodeJobs(ENCODES, 420)
+    requests = jobs.requests + list(out.requests)
+    order = np.random.default_rng(406).permutation(len(requests))
+    mixed = [requests[i] for i in order]
+    arrivals = np.cumsum(np.random.default_rng(407).exponential(30_000.0, len(mixed))).astype(np.uint64)
+    server = api.Server(resident.set, api.server_options(streams=3, max_batch=6, admission_bytes=8 << 20))
+    st, latency, statuses, written, stats = server.replay(mixed, arrivals)
+    assert st == ts.Status.success and (statuses == 0).all() and stats.failed == 0
+    pos = {int(i): k for k, i in enumerate(order)}
+    mix_files = [jobs.file(k, int(written[pos[k]])) for k in range(len(ENCODES))]
+    want_bytes = sum(p.nbytes for p in jobs.pixels) + sum(
+        r[1][2] * r[1][3] * resident.pixels[r[0]].itemsize for r in specs)
+    assert stats.decoded_bytes == want_bytes
+    for k in range(len(ENCODES)):
+        jobs.outs[k].fill(0xA5)
+        st, statuses, written = server.run_batch([jobs.requests[k]])
+        assert st == ts.Status.success and statuses[0] == 0
+        assert jobs.file(k, int(written[0])) == mix_files[k], (k, ENCODES[k])
+
+
+def test_encode_request_checks(resident):
+    server = api.Server(resident.set, api.server_options(max_tasks=64))
+    src = api.DeviceMemory(1 << 20)
+    dst = api.DeviceMemory(1 << 20)
+    view = api.device_view(src, ts.PixelType.uint16, 100, 100, const=True)
+    st, bound = api.compress_bound(view.image, api.TileShape(64, 64))
+    ok = api.encode_request(view, (64, 64), api.MutableBytes(dst.ptr, bound))
+    assert server.check(ok) == ts.Status.success
+    small = api.encode_request(view, (64, 64), api.MutableBytes(dst.ptr, bound - 4))
+    assert server.check(small) == ts.Status.buffer_too_small
+    misaligned = api.encode_request(view, (64, 64), api.MutableBytes(dst.ptr + 2, bound))
+    assert server.check(misaligned) == ts.Status.invalid_argument
+    bad_tile = api.encode_request(view, (0, 64), api.MutableBytes(dst.ptr, bound))
+    assert server.check(bad_tile) == ts.Status.invalid_argument
+    big = api.device_view(src, ts.PixelType.uint16, 700, 700, const=True)
+    st, big_bound = api.compress_bound(big.image, api.TileShape(64, 64))
+    too_many = api.encode_request(big, (64, 64), api.MutableBytes(dst.ptr, big_bound))
+    assert server.check(too_many) == ts.Status.resource_limit  # 121 tiles > 64
+    # an encode output must not overlap its own input or another output
+    into_input = api.encode_request(view, (64, 64), api.MutableBytes(src.ptr + 4096, bound))
+    st, *_ = server.run_batch([into_input])
+    assert st == ts.Status.invalid_argument
+    st, *_ = server.run_batch([ok, ok])
+    assert st == ts.Status.invalid_argument

@@ -133,8 +133,9 @@ class ArchiveInfo(C.Structure):
     _fields_ = [("image", ImageSpec), ("tile", TileShape), ("tile_count", C.c_uint64), ("file_bytes", C.c_uint64)]
 
 
-class RegionRequest(C.Structure):
-    _fields_ = [("archive", C.c_uint32), ("calibrated", C.c_uint32), ("roi", Roi), ("output", ImageView),
+class ServeRequest(C.Structure):
+    _fields_ = [("kind", C.c_uint32), ("archive", C.c_uint32), ("calibrated", C.c_uint32), ("tile", TileShape),
+                ("roi", Roi), ("output", ImageView), ("input", ConstImageView), ("compressed", MutableBytes),
                 ("offset", C.c_float), ("gain", C.c_float)]
 
 
@@ -158,7 +159,7 @@ RECORDS = {
     "ts_parameter": Parameter, "ts_calibration": Calibration, "ts_tile_mapping": TileMapping,
     "ts_compressed_input": CompressedInput, "ts_batch_item": BatchItem,
     "ts_batch_request": BatchRequest, "ts_launch_shape": LaunchShape, "ts_queue_stats": QueueStats,
-    "ts_archive_info": ArchiveInfo, "ts_region_request": RegionRequest, "ts_server_options": ServerOptions,
+    "ts_archive_info": ArchiveInfo, "ts_serve_request": ServeRequest, "ts_server_options": ServerOptions,
     "ts_server_stats": ServerStats,
 }
 
@@ -211,9 +212,10 @@ PROTOTYPES = {
     "ts_server_default_options": (None, [_P(ServerOptions)]),
     "ts_server_create": (_u32, [_vp, _P(ServerOptions), _P(_vp)]),
     "ts_server_destroy": (None, [_vp]),
-    "ts_server_check": (_u32, [_vp, _P(RegionRequest)]),
-    "ts_server_replay": (_u32, [_vp, _P(RegionRequest), _P(_u64), _u64, _P(_u64), _P(_u32), _P(ServerStats)]),
-    "ts_server_run_batch": (_u32, [_vp, _P(RegionRequest), _u64, _P(_u32)]),
+    "ts_server_check": (_u32, [_vp, _P(ServeRequest)]),
+    "ts_server_replay": (_u32, [_vp, _P(ServeRequest), _P(_u64), _u64, _P(_u64), _P(DeviceResult),
+                                _P(ServerStats)]),
+    "ts_server_run_batch": (_u32, [_vp, _P(ServeRequest), _u64, _P(DeviceResult)]),
     "ts_hip_malloc": (_int, [_u64, _P(_vp)]),
     "ts_hip_free": (_int, [_vp]),
     "ts_hip_host_malloc": (_int, [_u64, _P(_vp)]),

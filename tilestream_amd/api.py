@@ -17,7 +17,7 @@ import numpy as np
 from . import _native as N
 from ._native import (ArchiveInfo, BatchItem, BatchRequest, Calibration, CompressedInput, ConstBytes,
                       ConstImageView, DeviceResult, ImageSpec, ImageView, LaunchShape, MutableBytes,
-                      Parameter, ParseLimits, QueueStats, RegionRequest, Roi, SegmentMetadata,
+                      Parameter, ParseLimits, QueueStats, Roi, SegmentMetadata, ServeRequest,
                       ServerOptions, ServerStats, TileMapping, TileMetadata, TileShape, WorkspaceSpec)
 
 MAX_BYTES = (1 << 63) - 1
@@ -538,12 +538,19 @@ def server_options(streams=None, max_batch=None, max_tasks=None, admission_bytes
     return o
 
 
-def region_request(archive: int, roi, output: ImageView, calibrated=False, offset=0.0, gain=1.0) -> RegionRequest:
-    return RegionRequest(archive, 1 if calibrated else 0, Roi(*roi), output, offset, gain)
+def region_request(archive: int, roi, output: ImageView, calibrated=False, offset=0.0, gain=1.0) -> ServeRequest:
+    """Decode of `roi` of a resident archive into `output` (FP32 (x - offset) * gain when calibrated)."""
+    return ServeRequest(kind=0, archive=archive, calibrated=1 if calibrated else 0, roi=Roi(*roi), output=output,
+                        offset=offset, gain=gain)
+
+
+def encode_request(input: ConstImageView, tile, compressed: MutableBytes) -> ServeRequest:
+    """Canonical encode of the device image `input` with tile shape `tile` into `compressed`."""
+    return ServeRequest(kind=1, tile=TileShape(*tile), input=input, compressed=compressed, gain=1.0)
 
 
 class Server:
-    """Continuous batching of region requests over resident archives."""
+    """Continuous batching of region decodes and encodes over resident archives."""
 
     def __init__(self, archives: ArchiveSet, options: Optional[ServerOptions] = None):
         handle = C.c_void_p(0)
@@ -554,29 +561,35 @@ class Server:
         self.handle = handle.value
         self._archives = archives
 
-    def check(self, request: RegionRequest) -> Status:
+    def check(self, request: ServeRequest) -> Status:
         return Status(N.lib().ts_server_check(self.handle, C.byref(request)))
 
-    def replay(self, requests: Sequence[RegionRequest], arrival_ns):
-        """Open-loop replay; returns (status, latency_ns, statuses, stats)."""
+    @staticmethod
+    def _results(n: int):
+        res = np.zeros(max(1, n), dtype=[("status", "<u4"), ("reserved", "<u4"), ("bytes_written", "<u8")])
+        return res, res.ctypes.data_as(C.POINTER(DeviceResult))
+
+    def replay(self, requests: Sequence[ServeRequest], arrival_ns):
+        """Open-loop replay; returns (status, latency_ns, statuses, bytes_written, stats)."""
         n = len(requests)
-        arr = (RegionRequest * max(1, n))(*requests)
+        arr = (ServeRequest * max(1, n))(*requests)
         arrivals = np.ascontiguousarray(arrival_ns, dtype=np.uint64)
         assert arrivals.size == n
         latency = np.zeros(max(1, n), np.uint64)
-        statuses = np.zeros(max(1, n), np.uint32)
+        res, res_ptr = self._results(n)
         stats = ServerStats()
         st = Status(N.lib().ts_server_replay(self.handle, arr, arrivals.ctypes.data_as(C.POINTER(C.c_uint64)), n,
-                                             latency.ctypes.data_as(C.POINTER(C.c_uint64)),
-                                             statuses.ctypes.data_as(C.POINTER(C.c_uint32)), C.byref(stats)))
-        return st, latency[:n], statuses[:n], stats
+                                             latency.ctypes.data_as(C.POINTER(C.c_uint64)), res_ptr,
+                                             C.byref(stats)))
+        return st, latency[:n], res["status"][:n].copy(), res["bytes_written"][:n].copy(), stats
 
-    def run_batch(self, requests: Sequence[RegionRequest]):
+    def run_batch(self, requests: Sequence[ServeRequest]):
+        """All requests at once; returns (status, statuses, bytes_written)."""
         n = len(requests)
-        arr = (RegionRequest * max(1, n))(*requests)
-        statuses = np.zeros(max(1, n), np.uint32)
-        st = Status(N.lib().ts_server_run_batch(self.handle, arr, n, statuses.ctypes.data_as(C.POINTER(C.c_uint32))))
-        return st, statuses[:n]
+        arr = (ServeRequest * max(1, n))(*requests)
+        res, res_ptr = self._results(n)
+        st = Status(N.lib().ts_server_run_batch(self.handle, arr, n, res_ptr))
+        return st, res["status"][:n].copy(), res["bytes_written"][:n].copy()
 
     def close(self) -> None:
         if self.handle:

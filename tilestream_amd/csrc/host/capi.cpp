@@ -65,11 +65,15 @@ struct ts_server {
 };
 
 TS_SAME_LAYOUT(ts_archive_info, hip::ArchiveInfo);
-TS_SAME_LAYOUT(ts_region_request, hip::RegionRequest);
+TS_SAME_LAYOUT(ts_serve_request, hip::ServeRequest);
 TS_SAME_LAYOUT(ts_server_options, hip::ServerOptions);
 TS_SAME_LAYOUT(ts_server_stats, hip::ServerStats);
-static_assert(offsetof(ts_region_request, output) == offsetof(hip::RegionRequest, output));
-static_assert(offsetof(ts_region_request, offset) == offsetof(hip::RegionRequest, offset));
+static_assert(offsetof(ts_serve_request, tile) == offsetof(hip::ServeRequest, tile));
+static_assert(offsetof(ts_serve_request, roi) == offsetof(hip::ServeRequest, roi));
+static_assert(offsetof(ts_serve_request, output) == offsetof(hip::ServeRequest, output));
+static_assert(offsetof(ts_serve_request, input) == offsetof(hip::ServeRequest, input));
+static_assert(offsetof(ts_serve_request, compressed) == offsetof(hip::ServeRequest, compressed));
+static_assert(offsetof(ts_serve_request, offset) == offsetof(hip::ServeRequest, offset));
 static_assert(offsetof(ts_server_options, admission_bytes) == offsetof(hip::ServerOptions, admission_bytes));
 static_assert(offsetof(ts_server_options, shape) == offsetof(hip::ServerOptions, shape));
 static_assert(offsetof(ts_archive_info, tile_count) == offsetof(hip::ArchiveInfo, tile_count));
@@ -341,20 +345,22 @@ uint32_t ts_server_create(const ts_archive_set *set, const ts_server_options *op
     }
 }
 void ts_server_destroy(ts_server *server) { delete server; }
-uint32_t ts_server_check(const ts_server *server, const ts_region_request *request) {
+uint32_t ts_server_check(const ts_server *server, const ts_serve_request *request) {
     if (server == nullptr || request == nullptr) return code(Status::invalid_argument);
-    return code(server->server->check(as<hip::RegionRequest>(*request)));
+    return code(server->server->check(as<hip::ServeRequest>(*request)));
 }
-uint32_t ts_server_replay(ts_server *server, const ts_region_request *requests, const uint64_t *arrival_ns,
-                          uint64_t count, uint64_t *latency_ns, uint32_t *statuses, ts_server_stats *stats) {
+uint32_t ts_server_replay(ts_server *server, const ts_serve_request *requests, const uint64_t *arrival_ns,
+                          uint64_t count, uint64_t *latency_ns, ts_device_result *results, ts_server_stats *stats) {
     if (server == nullptr || stats == nullptr) return code(Status::invalid_argument);
-    return code(server->server->replay(reinterpret_cast<const hip::RegionRequest *>(requests), arrival_ns, count,
-                                       latency_ns, statuses, as<hip::ServerStats>(*stats)));
+    return code(server->server->replay(reinterpret_cast<const hip::ServeRequest *>(requests), arrival_ns, count,
+                                       latency_ns, reinterpret_cast<DeviceResult *>(results),
+                                       as<hip::ServerStats>(*stats)));
 }
-uint32_t ts_server_run_batch(ts_server *server, const ts_region_request *requests, uint64_t count,
-                             uint32_t *statuses) {
+uint32_t ts_server_run_batch(ts_server *server, const ts_serve_request *requests, uint64_t count,
+                             ts_device_result *results) {
     if (server == nullptr) return code(Status::invalid_argument);
-    return code(server->server->run_batch(reinterpret_cast<const hip::RegionRequest *>(requests), count, statuses));
+    return code(server->server->run_batch(reinterpret_cast<const hip::ServeRequest *>(requests), count,
+                                          reinterpret_cast<DeviceResult *>(results)));
 }
 
 int ts_hip_malloc(uint64_t bytes, void **data) { return hipMalloc(data, bytes); }
@@ -430,7 +436,7 @@ uint64_t ts_abi_sizeof(const char *name) {
     TS_SIZE(ts_launch_shape)
     TS_SIZE(ts_queue_stats)
     TS_SIZE(ts_archive_info)
-    TS_SIZE(ts_region_request)
+    TS_SIZE(ts_serve_request)
     TS_SIZE(ts_server_options)
     TS_SIZE(ts_server_stats)
 #undef TS_SIZE

@@ -1,5 +1,5 @@
 // Serving resident archives (tilestream/serve.h): archive tile tables planned
-// once, region batches described by a few bytes per request, continuous
+// once, region and encode requests described by a few bytes each, continuous
 // batching over several streams.
 #include "plan.h"
 
@@ -167,45 +167,83 @@ namespace {
 
 // One batch in flight: a stream, the batch's tables staged in pinned host
 // memory and copied to the device by a kernel, the final results written back
-// into pinned host memory by a kernel, and the event that marks its end.
+// into pinned host memory by kernels, and the event that marks its end.
 //
-// Table layout (host and device alike): work counter | DeviceResult[n] |
-// tsk_decode_image[n] | tsk_request[n] | request index.
+// Table layout (host and device alike): work counter | DeviceResult[n]
+// (region requests first, then encode requests) | tsk_decode_image[regions] |
+// tsk_request[regions] | request of every decode task | tsk_encode_image[encodes]
+// (uint16 images, then float32) | encode window entries.
 struct Slot {
     hipStream_t stream = nullptr;
     hipEvent_t done = nullptr;
     std::uint8_t *host = nullptr;      // tables, then the published results
     std::uint8_t *host_view = nullptr; // the same memory as the device addresses it
     std::uint8_t *device = nullptr;
+    std::uint8_t *scratch = nullptr;   // encoder workspace (rows, tiles, offsets, carry)
     std::uint64_t published = 0;       // offset of the published results in `host`
-    std::uint64_t first = 0, count = 0; // requests [first, first + count) of the current replay
-    std::uint64_t bytes = 0;            // output bytes admitted
+    std::vector<std::uint64_t> order;  // request index of each result of the current batch
+    std::uint64_t bytes = 0;           // output bytes admitted
     bool busy = false;
 };
 
 constexpr std::uint64_t kCounterBytes = 256;
+constexpr std::uint32_t kMaxTileRows = 64;
 
 struct Layout {
-    std::uint64_t results, images, requests, index, total;
+    std::uint64_t results, images, requests, index, encode, entries, total;
 };
 
-Layout layout_for(std::uint64_t requests, std::uint64_t tasks) {
+Layout layout_for(std::uint64_t regions, std::uint64_t encodes, std::uint64_t tasks, std::uint64_t entries) {
     Layout l{};
     l.results = kCounterBytes;
-    l.images = align_up(l.results + requests * sizeof(DeviceResult), 256);
-    l.requests = align_up(l.images + requests * sizeof(tsk_decode_image), 256);
-    l.index = align_up(l.requests + requests * sizeof(tsk_request), 256);
-    l.total = l.index + tasks * sizeof(std::uint32_t);
+    l.images = align_up(l.results + (regions + encodes) * sizeof(DeviceResult), 256);
+    l.requests = align_up(l.images + regions * sizeof(tsk_decode_image), 256);
+    l.index = align_up(l.requests + regions * sizeof(tsk_request), 256);
+    l.encode = align_up(l.index + tasks * sizeof(std::uint32_t), 256);
+    l.entries = align_up(l.encode + encodes * sizeof(tsk_encode_image), 256);
+    l.total = align_up(l.entries + entries * sizeof(tsk_window_entry), 16);
     return l;
 }
 
-// The tiles a request decodes, and its output bytes.
-struct Extent {
-    std::uint32_t col0 = 0, row0 = 0, cols = 0, rows = 0;
-    std::uint64_t tasks = 0;
-    std::uint64_t bytes = 0;   // output bytes
-    std::uint64_t decoded = 0; // region pixels x stored sample bytes
+// Encoder workspace per slot: one window of tiles at a time.
+struct Scratch {
+    static constexpr std::uint64_t rows = 0;
+    static constexpr std::uint64_t tiles = rows + std::uint64_t{detail::window_tiles} * kMaxTileRows;
+    static constexpr std::uint64_t offsets = tiles + std::uint64_t{detail::window_tiles} * 8;
+    static constexpr std::uint64_t carry = offsets + (std::uint64_t{detail::window_tiles} + 1) * 8;
+    static constexpr std::uint64_t total = carry + 256;
 };
+
+// What a request costs a batch.
+struct Extent {
+    std::uint32_t col0 = 0, row0 = 0, cols = 0, rows = 0; // region: its tiles
+    std::uint64_t tasks = 0;   // tiles decoded or encoded
+    std::uint64_t bytes = 0;   // output bytes (encode: the compress bound)
+    std::uint64_t decoded = 0; // region pixels x stored sample bytes, or encoded input bytes
+};
+
+// Splits the tiles of consecutive images into windows of at most
+// window_tiles tiles; returns the entry index where each window starts.
+void split_windows(const std::vector<std::uint64_t> &tiles, std::vector<tsk_window_entry> &entries,
+                   std::vector<std::pair<std::size_t, std::uint32_t>> &windows) {
+    std::uint32_t used = detail::window_tiles;
+    for (std::size_t image = 0; image < tiles.size(); ++image) {
+        for (std::uint64_t start = 0; start < tiles[image];) {
+            if (used == detail::window_tiles) {
+                windows.push_back({entries.size(), 0});
+                used = 0;
+            }
+            const auto take = static_cast<std::uint32_t>(
+                std::min<std::uint64_t>(tiles[image] - start, detail::window_tiles - used));
+            std::uint32_t flags = start == 0 ? 0u : 1u;
+            if (start + take == tiles[image]) flags |= 2u;
+            entries.push_back({static_cast<std::uint32_t>(image), used, take, flags, start});
+            windows.back().second += take;
+            used += take;
+            start += take;
+        }
+    }
+}
 
 } // namespace
 
@@ -215,6 +253,7 @@ struct Server::Impl {
     int device = 0;
     int cus = 0;
     int per_cu[5] = {};
+    bool pageable = false;
     std::vector<Slot> slots;
     std::vector<Extent> extents; // per request of the current replay
 
@@ -225,12 +264,16 @@ struct Server::Impl {
             if (s.stream != nullptr) static_cast<void>(hipStreamDestroy(s.stream));
             if (s.host != nullptr) static_cast<void>(hipHostFree(s.host));
             if (s.device != nullptr) static_cast<void>(hipFree(s.device));
+            if (s.scratch != nullptr) static_cast<void>(hipFree(s.scratch));
         }
     }
 
-    Status extent(const RegionRequest &r, Extent &e) const noexcept;
-    Status launch(Slot &slot, const RegionRequest *requests, std::uint64_t first, std::uint64_t count,
-                  std::uint64_t tasks) noexcept;
+    [[nodiscard]] Layout capacity() const noexcept {
+        const std::uint64_t n = options.max_batch;
+        return layout_for(n, n, options.max_tasks, n + 2 * (options.max_tasks / detail::window_tiles + 2));
+    }
+    Status extent(const ServeRequest &r, Extent &e) const noexcept;
+    Status launch(Slot &slot, const ServeRequest *requests, std::uint64_t first, std::uint64_t count) noexcept;
 };
 
 Server::Server(std::unique_ptr<Impl> impl) noexcept : impl_(std::move(impl)) {}
@@ -248,14 +291,16 @@ Status Server::create(const ArchiveSet &archives, const ServerOptions &options, 
         auto impl = std::make_unique<Impl>();
         impl->set = set;
         impl->options = options;
+        impl->pageable = set->pageable;
         if (hipGetDevice(&impl->device) != hipSuccess || impl->device != set->device ||
             hipDeviceGetAttribute(&impl->cus, hipDeviceAttributeMultiprocessorCount, impl->device) != hipSuccess)
             return detail::hip_failure();
         for (std::uint32_t kind = 0; kind < 5; ++kind)
             if (tsk_decode_occupancy(kind, sh.group, sh.block, sh.depth, &impl->per_cu[kind]) != hipSuccess)
                 return detail::hip_failure();
-        if (tsk_preload(1) != hipSuccess || tsk_preload(3) != hipSuccess) return detail::hip_failure();
-        const Layout cap = layout_for(options.max_batch, options.max_tasks);
+        if (tsk_preload(0) != hipSuccess || tsk_preload(1) != hipSuccess || tsk_preload(3) != hipSuccess)
+            return detail::hip_failure();
+        const Layout cap = impl->capacity();
         impl->slots.resize(options.streams);
         for (Slot &s : impl->slots) {
             if (hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking) != hipSuccess) {
@@ -266,9 +311,9 @@ Status Server::create(const ArchiveSet &archives, const ServerOptions &options, 
                 s.done = nullptr;
                 return detail::hip_failure();
             }
-            void *h = nullptr, *hv = nullptr, *d = nullptr;
+            void *h = nullptr, *hv = nullptr, *d = nullptr, *w = nullptr;
             s.published = align_up(cap.total, 256);
-            const std::uint64_t host_bytes = s.published + std::uint64_t{options.max_batch} * sizeof(DeviceResult);
+            const std::uint64_t host_bytes = s.published + 2 * std::uint64_t{options.max_batch} * sizeof(DeviceResult);
             // Coherent: the device reads the tables and writes the results over
             // the bus while the host polls.
             if (hipHostMalloc(&h, host_bytes, hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess)
@@ -278,6 +323,9 @@ Status Server::create(const ArchiveSet &archives, const ServerOptions &options, 
             s.host_view = static_cast<std::uint8_t *>(hv);
             if (hipMalloc(&d, align_up(cap.total, 256)) != hipSuccess) return detail::hip_failure();
             s.device = static_cast<std::uint8_t *>(d);
+            if (hipMalloc(&w, Scratch::total) != hipSuccess) return detail::hip_failure();
+            s.scratch = static_cast<std::uint8_t *>(w);
+            s.order.reserve(options.max_batch);
         }
         server.reset(new Server(std::move(impl)));
         return Status::success;
@@ -288,7 +336,25 @@ Status Server::create(const ArchiveSet &archives, const ServerOptions &options, 
     }
 }
 
-Status Server::Impl::extent(const RegionRequest &r, Extent &e) const noexcept {
+Status Server::Impl::extent(const ServeRequest &r, Extent &e) const noexcept {
+    if (r.kind == RequestKind::encode) {
+        // The checks of a compress item (reference plan.cpp:351-465, draft_compress here).
+        std::uint64_t bound = 0;
+        if (const Status st = host::check_view(r.input); st != Status::success) return st;
+        if (const Status st = compress_bound(r.input.image, r.tile, bound); st != Status::success) return st;
+        if (const Status st = host::byte_range(r.compressed.data, r.compressed.size); st != Status::success) return st;
+        if (r.compressed.data == nullptr || reinterpret_cast<std::uintptr_t>(r.compressed.data) % 4 != 0)
+            return Status::invalid_argument;
+        if (r.compressed.size < bound) return Status::buffer_too_small;
+        if (r.tile.height > kMaxTileRows) return Status::invalid_argument;
+        e.tasks = host::ceil_div(r.input.image.width, r.tile.width) * host::ceil_div(r.input.image.height, r.tile.height);
+        if (e.tasks > options.max_tasks) return Status::resource_limit;
+        const std::uint64_t s = r.input.image.type == PixelType::uint16 ? 2 : 4;
+        e.bytes = bound;
+        e.decoded = std::uint64_t{r.input.image.width} * r.input.image.height * s;
+        return Status::success;
+    }
+    if (r.kind != RequestKind::region) return Status::invalid_argument;
     if (r.archive >= set->committed || r.calibrated > 1) return Status::invalid_argument;
     const Archive &a = set->archives[r.archive];
     const ImageSpec &image = a.info.image;
@@ -314,28 +380,70 @@ Status Server::Impl::extent(const RegionRequest &r, Extent &e) const noexcept {
     return Status::success;
 }
 
-Status Server::check(const RegionRequest &request) const noexcept {
+Status Server::check(const ServeRequest &request) const noexcept {
     Extent e;
     return impl_->extent(request, e);
 }
 
-// Stages the batch's tables, uploads them with one copy, decodes in request
-// mode, completes the results and copies them back, then marks the end.
-Status Server::Impl::launch(Slot &slot, const RegionRequest *requests, std::uint64_t first, std::uint64_t count,
-                            std::uint64_t tasks) noexcept {
-    const Layout l = layout_for(count, tasks);
+// Stages the batch's tables, copies them to the device with one kernel, runs
+// the region decode in request mode and the encode windows, and copies the
+// final results into pinned memory; then marks the end.
+Status Server::Impl::launch(Slot &slot, const ServeRequest *requests, std::uint64_t first, std::uint64_t count) noexcept {
+    std::uint64_t regions = 0, tasks = 0;
+    std::uint64_t encodes[2] = {0, 0}; // uint16, float32
+    std::vector<std::uint64_t> enc_tiles[2];
+    for (std::uint64_t k = 0; k < count; ++k) {
+        const ServeRequest &r = requests[first + k];
+        if (r.kind == RequestKind::region) {
+            ++regions;
+            tasks += extents[first + k].tasks;
+        } else {
+            const int g = r.input.image.type == PixelType::uint16 ? 0 : 1;
+            ++encodes[g];
+            enc_tiles[g].push_back(extents[first + k].tasks);
+        }
+    }
+    std::vector<tsk_window_entry> entries;
+    std::vector<std::pair<std::size_t, std::uint32_t>> windows[2];
+    split_windows(enc_tiles[0], entries, windows[0]);
+    const std::size_t group1_entries = entries.size();
+    split_windows(enc_tiles[1], entries, windows[1]);
+    const Layout l = layout_for(regions, encodes[0] + encodes[1], tasks, entries.size());
+
     std::uint8_t *h = slot.host;
     std::memset(h, 0, l.images); // work counter and results start at zero
     auto *images = reinterpret_cast<tsk_decode_image *>(h + l.images);
     auto *reqs = reinterpret_cast<tsk_request *>(h + l.requests);
     auto *index = reinterpret_cast<std::uint32_t *>(h + l.index);
+    auto *enc = reinterpret_cast<tsk_encode_image *>(h + l.encode);
+    if (!entries.empty())
+        std::memcpy(h + l.entries, entries.data(), entries.size() * sizeof(tsk_window_entry));
+    slot.order.assign(regions + encodes[0] + encodes[1], 0);
     bool u16 = false, f32 = false, cal = false, plain = false;
-    std::uint64_t task = 0;
+    std::uint64_t task = 0, j = 0, e_next[2] = {0, encodes[0]};
     for (std::uint64_t k = 0; k < count; ++k) {
-        const RegionRequest &r = requests[first + k];
+        const ServeRequest &r = requests[first + k];
         const Extent &e = extents[first + k];
+        if (r.kind == RequestKind::encode) {
+            const int g = r.input.image.type == PixelType::uint16 ? 0 : 1;
+            const std::uint64_t at = e_next[g]++;
+            tsk_encode_image &im = enc[at];
+            im = tsk_encode_image{};
+            im.in = static_cast<const std::uint8_t *>(r.input.data);
+            im.out = static_cast<std::uint8_t *>(r.compressed.data);
+            im.in_pitch = r.input.pitch_bytes;
+            im.tile_count = e.tasks;
+            im.width = r.input.image.width;
+            im.height = r.input.image.height;
+            im.tile_columns = static_cast<std::uint32_t>(host::ceil_div(r.input.image.width, r.tile.width));
+            im.tile_w = r.tile.width;
+            im.tile_h = r.tile.height;
+            im.type = static_cast<std::uint32_t>(r.input.image.type);
+            slot.order[regions + at] = first + k;
+            continue;
+        }
         const Archive &a = set->archives[r.archive];
-        tsk_decode_image &im = images[k];
+        tsk_decode_image &im = images[j];
         im = tsk_decode_image{};
         im.src = a.bytes;
         im.out = static_cast<std::uint8_t *>(r.output.data);
@@ -354,7 +462,7 @@ Status Server::Impl::launch(Slot &slot, const RegionRequest *requests, std::uint
         im.calibrated = static_cast<std::uint8_t>(r.calibrated);
         im.offset = r.calibrated ? r.offset : 0.0F;
         im.gain = r.calibrated ? r.gain : 1.0F;
-        tsk_request &q = reqs[k];
+        tsk_request &q = reqs[j];
         q.first_task = task;
         q.tile_base = static_cast<std::uint32_t>(a.tile_base);
         q.tile_columns = a.tile_columns;
@@ -362,56 +470,90 @@ Status Server::Impl::launch(Slot &slot, const RegionRequest *requests, std::uint
         q.row0 = e.row0;
         q.cols = e.cols;
         q.reserved = 0;
-        std::fill_n(index + task, e.tasks, static_cast<std::uint32_t>(k));
+        std::fill_n(index + task, e.tasks, static_cast<std::uint32_t>(j));
         task += e.tasks;
+        slot.order[j] = first + k;
+        ++j;
         u16 = u16 || a.info.image.type == PixelType::uint16;
         f32 = f32 || a.info.image.type == PixelType::float32;
         cal = cal || r.calibrated != 0;
         plain = plain || r.calibrated == 0;
     }
-    std::uint32_t kind = TSK_KIND_MIXED;
-    if (u16 != f32 && cal != plain)
-        kind = u16 ? (cal ? TSK_KIND_U16_CAL : TSK_KIND_U16) : (cal ? TSK_KIND_F32_CAL : TSK_KIND_F32);
 
-    const LaunchShape &sh = options.shape;
-    std::uint64_t resident = static_cast<std::uint64_t>(cus) * static_cast<std::uint64_t>(std::max(1, per_cu[kind]));
-    std::uint64_t grid = sh.grid != 0 ? sh.grid : std::min<std::uint64_t>(resident, tasks);
-    const bool queued = sh.chunk != 0 && tasks > grid;
-
-    tsk_decode_args args{};
-    args.images = reinterpret_cast<const tsk_decode_image *>(slot.device + l.images);
-    args.tasks = set->device_tiles;
-    args.rows = set->device_rows;
-    args.results = slot.device + l.results;
-    args.queue = queued ? reinterpret_cast<std::uint64_t *>(slot.device) : nullptr;
-    args.task_count = tasks;
-    args.row_count = set->row_count;
-    args.image_count = static_cast<std::uint32_t>(count);
-    args.kind = kind;
-    args.group = sh.group;
-    args.block = sh.block;
-    args.grid = static_cast<std::uint32_t>(std::max<std::uint64_t>(1, grid));
-    args.chunk = queued ? sh.chunk : 0;
-    args.depth = sh.depth;
-    args.request_count = static_cast<std::uint32_t>(count);
-    args.requests = reinterpret_cast<const tsk_request *>(slot.device + l.requests);
-    args.request_index = reinterpret_cast<const std::uint32_t *>(slot.device + l.index);
     // Kernels read the staged tables from pinned memory and write the final
     // results back into it: they start sooner than runtime copies.
-    if (tsk_copy_blocks(slot.device, slot.host_view, (l.total + 15) / 16, slot.stream) != hipSuccess ||
-        tsk_decode(&args, slot.stream) != hipSuccess ||
-        tsk_results_publish(args.images, static_cast<std::uint32_t>(count), slot.device + l.results,
-                            slot.host_view + slot.published, slot.stream) != hipSuccess ||
-        hipEventRecord(slot.done, slot.stream) != hipSuccess)
+    if (tsk_copy_blocks(slot.device, slot.host_view, l.total / 16, slot.stream) != hipSuccess)
         return detail::hip_failure();
-    slot.first = first;
-    slot.count = count;
+    if (regions != 0) {
+        std::uint32_t kind = TSK_KIND_MIXED;
+        if (u16 != f32 && cal != plain)
+            kind = u16 ? (cal ? TSK_KIND_U16_CAL : TSK_KIND_U16) : (cal ? TSK_KIND_F32_CAL : TSK_KIND_F32);
+        const LaunchShape &sh = options.shape;
+        const std::uint64_t resident =
+            static_cast<std::uint64_t>(cus) * static_cast<std::uint64_t>(std::max(1, per_cu[kind]));
+        const std::uint64_t grid = sh.grid != 0 ? sh.grid : std::min<std::uint64_t>(resident, tasks);
+        const bool queued = sh.chunk != 0 && tasks > grid;
+        tsk_decode_args args{};
+        args.images = reinterpret_cast<const tsk_decode_image *>(slot.device + l.images);
+        args.tasks = set->device_tiles;
+        args.rows = set->device_rows;
+        args.results = slot.device + l.results;
+        args.queue = queued ? reinterpret_cast<std::uint64_t *>(slot.device) : nullptr;
+        args.task_count = tasks;
+        args.row_count = set->row_count;
+        args.image_count = static_cast<std::uint32_t>(regions);
+        args.kind = kind;
+        args.group = sh.group;
+        args.block = sh.block;
+        args.grid = static_cast<std::uint32_t>(std::max<std::uint64_t>(1, grid));
+        args.chunk = queued ? sh.chunk : 0;
+        args.depth = sh.depth;
+        args.request_count = static_cast<std::uint32_t>(regions);
+        args.requests = reinterpret_cast<const tsk_request *>(slot.device + l.requests);
+        args.request_index = reinterpret_cast<const std::uint32_t *>(slot.device + l.index);
+        if (tsk_decode(&args, slot.stream) != hipSuccess ||
+            tsk_results_publish(args.images, static_cast<std::uint32_t>(regions), args.results,
+                                slot.host_view + slot.published, slot.stream) != hipSuccess)
+            return detail::hip_failure();
+    }
+    for (int g = 0; g < 2; ++g) {
+        if (encodes[g] == 0) continue;
+        tsk_encode_args a{};
+        a.images = reinterpret_cast<const tsk_encode_image *>(slot.device + l.encode) + (g == 0 ? 0 : encodes[0]);
+        a.results = slot.device + l.results + (regions + (g == 0 ? 0 : encodes[0])) * sizeof(DeviceResult);
+        a.rows = slot.scratch + Scratch::rows;
+        a.tiles = reinterpret_cast<std::uint32_t *>(slot.scratch + Scratch::tiles);
+        a.offsets = reinterpret_cast<std::uint64_t *>(slot.scratch + Scratch::offsets);
+        a.carry = reinterpret_cast<std::uint64_t *>(slot.scratch + Scratch::carry);
+        a.max_rows = kMaxTileRows;
+        a.type = g == 0 ? TSK_UINT16 : TSK_FLOAT32;
+        const std::size_t base = g == 0 ? 0 : group1_entries;
+        const std::size_t end_entries = g == 0 ? group1_entries : entries.size();
+        for (std::size_t w = 0; w < windows[g].size(); ++w) {
+            const std::size_t e0 = windows[g][w].first;
+            const std::size_t e1 = w + 1 < windows[g].size() ? windows[g][w + 1].first : end_entries;
+            a.entries = reinterpret_cast<const tsk_window_entry *>(slot.device + l.entries) + e0;
+            a.entry_count = static_cast<std::uint32_t>(e1 - e0);
+            a.tile_count = windows[g][w].second;
+            a.parity = static_cast<std::uint32_t>(w & 1u);
+            if (tsk_encode_analyze(&a, slot.stream) != hipSuccess || tsk_encode_scan(&a, slot.stream) != hipSuccess ||
+                tsk_encode_pack(&a, slot.stream) != hipSuccess || tsk_encode_finish(&a, slot.stream) != hipSuccess)
+                return detail::hip_failure();
+        }
+        static_cast<void>(base);
+    }
+    if (encodes[0] + encodes[1] != 0 &&
+        tsk_copy_blocks(slot.host_view + slot.published + regions * sizeof(DeviceResult),
+                        slot.device + l.results + regions * sizeof(DeviceResult), encodes[0] + encodes[1],
+                        slot.stream) != hipSuccess)
+        return detail::hip_failure();
+    if (hipEventRecord(slot.done, slot.stream) != hipSuccess) return detail::hip_failure();
     slot.busy = true;
     return Status::success;
 }
 
-Status Server::replay(const RegionRequest *requests, const std::uint64_t *arrival_ns, std::uint64_t count,
-                      std::uint64_t *latency_ns, std::uint32_t *statuses, ServerStats &stats) noexcept {
+Status Server::replay(const ServeRequest *requests, const std::uint64_t *arrival_ns, std::uint64_t count,
+                      std::uint64_t *latency_ns, DeviceResult *results, ServerStats &stats) noexcept {
     Impl &s = *impl_;
     stats = ServerStats{};
     if (count == 0) return Status::success;
@@ -426,12 +568,23 @@ Status Server::replay(const RegionRequest *requests, const std::uint64_t *arriva
         writable.reserve(count);
         for (std::uint64_t i = 0; i < count; ++i) {
             if (i > 0 && arrival_ns[i] < arrival_ns[i - 1]) return Status::invalid_argument;
-            if (const Status st = s.extent(requests[i], s.extents[i]); st != Status::success) return st;
-            const RegionRequest &r = requests[i];
+            const ServeRequest &r = requests[i];
+            if (const Status st = s.extent(r, s.extents[i]); st != Status::success) return st;
+            if (r.kind == RequestKind::encode) {
+                if (const Status st = detail::accessible(r.input.data, s.device, s.pageable); st != Status::success)
+                    return st;
+                if (const Status st = detail::accessible(r.compressed.data, s.device, s.pageable); st != Status::success)
+                    return st;
+                const auto in = reinterpret_cast<std::uintptr_t>(r.input.data);
+                const auto out = reinterpret_cast<std::uintptr_t>(r.compressed.data);
+                readable.push_back({in, in + r.input.capacity_bytes});
+                writable.push_back({out, out + r.compressed.size});
+                continue;
+            }
             std::uint64_t span = 0;
             if (const Status st = image_span(r.output.image, r.output.pitch_bytes, span); st != Status::success)
                 return st;
-            if (const Status st = detail::accessible(r.output.data, s.device, s.set->pageable); st != Status::success)
+            if (const Status st = detail::accessible(r.output.data, s.device, s.pageable); st != Status::success)
                 return st;
             const auto at = reinterpret_cast<std::uintptr_t>(r.output.data);
             writable.push_back({at, at + span});
@@ -451,10 +604,6 @@ Status Server::replay(const RegionRequest *requests, const std::uint64_t *arriva
         std::uint64_t arrived = 0, dispatched = 0, finished = 0, in_flight = 0;
         std::uint64_t last_done = 0;
         Status failure = Status::success;
-        auto drain = [&] {
-            for (Slot &slot : s.slots)
-                if (slot.busy) static_cast<void>(hipStreamSynchronize(slot.stream));
-        };
         while (finished < count) {
             const std::uint64_t now = elapsed();
             while (arrived < count && arrival_ns[arrived] <= now) ++arrived;
@@ -469,14 +618,19 @@ Status Server::replay(const RegionRequest *requests, const std::uint64_t *arriva
                 }
                 const std::uint64_t t = elapsed();
                 const auto *res = reinterpret_cast<const volatile DeviceResult *>(slot.host + slot.published);
-                for (std::uint64_t k = 0; k < slot.count; ++k) {
-                    const std::uint64_t i = slot.first + k;
+                for (std::size_t k = 0; k < slot.order.size(); ++k) {
+                    const std::uint64_t i = slot.order[k];
+                    const Status st = res[k].status;
                     if (latency_ns != nullptr) latency_ns[i] = t - arrival_ns[i];
-                    if (statuses != nullptr) statuses[i] = static_cast<std::uint32_t>(res[k].status);
-                    if (res[k].status != Status::success) ++stats.failed;
+                    if (results != nullptr) {
+                        results[i].status = st;
+                        results[i].reserved = 0;
+                        results[i].bytes_written = res[k].bytes_written;
+                    }
+                    if (st != Status::success) ++stats.failed;
                     stats.decoded_bytes += s.extents[i].decoded;
                 }
-                finished += slot.count;
+                finished += slot.order.size();
                 in_flight -= slot.bytes;
                 last_done = t;
                 slot.busy = false;
@@ -497,7 +651,7 @@ Status Server::replay(const RegionRequest *requests, const std::uint64_t *arriva
                     ++n;
                 }
                 if (n == 0) break;
-                if (const Status st = s.launch(slot, requests, dispatched, n, tasks); st != Status::success) {
+                if (const Status st = s.launch(slot, requests, dispatched, n); st != Status::success) {
                     failure = st;
                     break;
                 }
@@ -515,7 +669,8 @@ Status Server::replay(const RegionRequest *requests, const std::uint64_t *arriva
             }
         }
         if (failure != Status::success) {
-            drain();
+            for (Slot &slot : s.slots)
+                if (slot.busy) static_cast<void>(hipStreamSynchronize(slot.stream));
             for (Slot &slot : s.slots) slot.busy = false;
             return failure;
         }
@@ -529,11 +684,11 @@ Status Server::replay(const RegionRequest *requests, const std::uint64_t *arriva
     }
 }
 
-Status Server::run_batch(const RegionRequest *requests, std::uint64_t count, std::uint32_t *statuses) noexcept {
+Status Server::run_batch(const ServeRequest *requests, std::uint64_t count, DeviceResult *results) noexcept {
     try {
         std::vector<std::uint64_t> arrivals(count, 0);
         ServerStats stats;
-        return replay(requests, arrivals.data(), count, nullptr, statuses, stats);
+        return replay(requests, arrivals.data(), count, nullptr, results, stats);
     } catch (const std::bad_alloc &) {
         return Status::resource_limit;
     }
