@@ -243,6 +243,84 @@ def roi_leg(args, threads: int, stream):
     }
 
 
+def encode_leg(args, batch, stream):
+    """GPU canonical encode of the decode batch's images (pixels resident in
+    HBM): one compress plan per (pixel type, tile shape), all six run back to
+    back per step.  Archive bytes are checked against the host codec's."""
+    from tilestream_amd import api
+    from tilestream_amd.workloads import _align
+    groups = {}
+    for i, e in enumerate(batch.entries):
+        groups.setdefault((int(e.ptype), tuple(e.tile)), []).append(i)
+    raw_off, total = {}, 0
+    for e_i, e in enumerate(batch.entries):
+        raw_off[e_i] = total
+        total += _align(e.raw_bytes)
+    raw = api.DeviceMemory(total)
+    for e_i, e in enumerate(batch.entries):
+        raw.upload(e.pixels, raw_off[e_i])
+    plans, keep, out_total = [], [], 0
+    bounds = {}
+    for (ptype, tile), members in sorted(groups.items()):
+        for i in members:
+            e = batch.entries[i]
+            st, bound = api.compress_bound(api.ImageSpec(ptype, e.width, e.height), api.TileShape(*tile))
+            assert st == api.Status.success
+            bounds[i] = (out_total, bound)
+            out_total += _align(bound)
+    out = api.DeviceMemory(out_total)
+    ws_bytes = 0
+    for (ptype, tile), members in sorted(groups.items()):
+        items = []
+        for i in members:
+            e = batch.entries[i]
+            it = api.BatchItem()
+            it.input = api.device_view(raw, e.ptype, e.width, e.height, offset=raw_off[i], capacity=e.raw_bytes,
+                                       const=True)
+            off, bound = bounds[i]
+            it.compressed_output = api.MutableBytes(out.ptr + off, bound)
+            items.append(it)
+        st, plan = api.prepare_batch(api.Operation.compress, items, api.TileShape(*tile), stream=stream)
+        assert st == api.Status.success, st
+        ws_bytes = max(ws_bytes, plan.workspace_bytes())
+        res = api.DeviceMemory(16 * len(members))
+        plans.append((plan, res, members))
+        keep.append(items)
+    ws = api.DeviceMemory(ws_bytes)
+
+    def step():
+        for plan, res, members in plans:
+            api.compress_async(plan, ws, res, len(members), stream)
+
+    for _ in range(args.warmup):
+        step()
+    ms = timed(stream, step, args.steps)
+    stream.synchronize()
+    ok, written_total = True, 0
+    for plan, res, members in plans:
+        status, written = api.read_results(res, len(members))
+        ok = ok and bool((status == 0).all())
+        for i, n in zip(members, written):
+            written_total += int(n)
+            ok = ok and int(n) == len(batch.entries[i].archive)
+    if not args.no_verify:
+        for i in range(0, len(batch.entries), 16):
+            off, _ = bounds[i]
+            ok = ok and out.download(len(batch.entries[i].archive), off).tobytes() == batch.entries[i].archive
+    raw_bytes = batch.decoded_bytes
+    alg = raw_bytes + written_total
+    log(f"encode: {ms:.4f} ms per step, {raw_bytes / (ms * 1e-3) / 1e9:.1f} GB/s of pixels, "
+        f"{alg / (ms * 1e-3) / 1e9:.1f} GB/s algorithmic, verified {ok}")
+    return {"workload": f"encode of the {len(batch.entries)} batch images, one plan per (pixel type, tile shape), "
+                        f"{len(plans)} plans per step", "ms_per_step": round(ms, 4),
+            "pixels_GBps": round(raw_bytes / (ms * 1e-3) / 1e9, 1),
+            "algorithmic_GBps": round(alg / (ms * 1e-3) / 1e9, 1),
+            "roofline_frac": round(alg / (ms * 1e-3) / 1e9 / HBM_PEAK_GBPS, 4),
+            "algorithmic_bytes": alg, "archive_bytes": written_total,
+            "note": "algorithmic bytes = pixels read once + archive bytes written",
+            "verified_bit_exact": ok}
+
+
 def serve_mix_leg(args, threads: int, stream):
     """Serving mix: Poisson arrivals of region decodes (70%: sides log-uniform in
     [32, 2048] clipped to the image, uniform origins), whole-image decodes (20%)
@@ -504,6 +582,8 @@ def main() -> int:
     p.add_argument("--qps", type=lambda v: [float(x) for x in v.split(",")], default=[100000.0, 400000.0],
                    help="Poisson arrival rates (requests/s) for the ROI latency runs")
     p.add_argument("--no-mix", action="store_true", help="skip the serving-mix leg")
+    p.add_argument("--no-encode", action="store_true", help="skip the encode leg")
+    p.add_argument("--encode-only", action="store_true", help="only the encode leg (of the decode batch's images)")
     p.add_argument("--mix-only", action="store_true", help="only the serving-mix leg")
     p.add_argument("--mix-images", type=int, default=24)
     p.add_argument("--mix-requests", type=int, default=3000)
@@ -549,6 +629,10 @@ def main() -> int:
     stream = api.Stream()
     n = len(batch.entries)
     results = api.DeviceMemory(16 * n)
+    if args.encode_only:
+        enc = encode_leg(args, batch, stream)
+        print(json.dumps({"metric": "GPU encode GB/s", "encode": enc}), flush=True)
+        return 0 if enc["verified_bit_exact"] else 3
 
     # ---- batched persistent path -------------------------------------------------
     items = [batch.item(i) for i in range(n)]
@@ -641,6 +725,7 @@ def main() -> int:
     del plans, pws, rec
     roi = None if args.no_roi else roi_leg(args, threads, stream)
     fused = None if args.no_fused else fused_leg(args, stream)
+    enc = None if args.no_encode else encode_leg(args, batch, stream)
     mix = None if args.no_mix else serve_mix_leg(args, threads, stream)
     if cpu:
         log(f"cpu baseline {cpu['value']} GB/s on {cpu['cores']} threads ({cpu['kind']})")
@@ -691,13 +776,14 @@ def main() -> int:
         "verified_bit_exact": verified,
         "roi": roi,
         "fused_real_shape": fused,
+        "encode": enc,
         "serve_mix": mix,
     }
     if roi is not None:
         out["p99_roi_latency_us"] = {str(int(r["qps"])): r["p99_us"] for r in roi["latency"]}
     print(json.dumps(out), flush=True)
     ok = verified in (None, True) and all(leg is None or leg["verified_bit_exact"] in (None, True)
-                                          for leg in (roi, mix))
+                                          for leg in (roi, mix, enc))
     return 0 if ok else 3
 
 

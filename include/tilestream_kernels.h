@@ -8,10 +8,10 @@
  * These launchers replace the reference's kernel launches:
  *   tsk_results_begin / tsk_results_complete  <- reference src/cuda/decoder.cu:307-326
  *   tsk_decode                                 <- reference src/cuda/decoder.cu:176-305, 328-339
- *   tsk_encode_analyze                         <- reference src/cuda/encoder.cu:62-138
- *   tsk_encode_scan                            <- reference src/cuda/encoder.cu:284-288 (CUB scan)
- *   tsk_encode_pack                            <- reference src/cuda/encoder.cu:175-238
- *   tsk_encode_finish                          <- reference src/cuda/encoder.cu:240-280
+ *   tsk_encode_window                          <- reference src/cuda/encoder.cu:62-138 (analyze),
+ *                                                 284-288 (scan), 175-238 (pack), 240-280 (finish):
+ *                                                 one single-pass kernel per window
+ *   tsk_encode_reset                           <- (no counterpart: clears the window state)
  *   tsk_calibrate / tsk_calibrate_report       <- reference src/cuda/calibrate.cu:13-62
  *   tsk_preload                                <- reference src/cuda/decoder.cu:341-349,
  *                                                 encoder.cu:329-349, calibrate.cu:64-71
@@ -132,15 +132,14 @@ typedef struct {
     const tsk_encode_image *images;
     const tsk_window_entry *entries;
     void *results;               /* DeviceResult per image */
-    uint8_t *rows;               /* workspace: per tile, tile_h row widths (1 byte each) */
-    uint32_t *tiles;             /* workspace: per tile {mode, length} */
-    uint64_t *offsets;           /* workspace: window prefix of payload lengths, tile_count + 1 */
+    uint64_t *state;             /* workspace: 2 x capacity look-back words, one buffer per window parity */
     uint64_t *carry;             /* workspace: 2 slots, output cursor of a continued image */
+    uint32_t capacity;           /* tiles per look-back buffer (the largest window) */
     uint32_t entry_count;
     uint32_t tile_count;         /* tiles in this window */
     uint32_t parity;             /* window number & 1 */
-    uint32_t max_rows;           /* row slots per tile in the workspace */
     uint32_t type;               /* TSK_UINT16 / TSK_FLOAT32 */
+    uint32_t reserved;
 } tsk_encode_args;
 
 /* Per-image descriptor of standalone calibration. */
@@ -175,10 +174,12 @@ int tsk_decode(const tsk_decode_args *args, void *stream);
 int tsk_decode_occupancy(uint32_t kind, uint32_t group, uint32_t block, uint32_t depth, int *blocks);
 /* Whether a (group, block, depth) combination has a compiled kernel. */
 int tsk_decode_supported(uint32_t group, uint32_t block, uint32_t depth);
-int tsk_encode_analyze(const tsk_encode_args *args, void *stream);
-int tsk_encode_scan(const tsk_encode_args *args, void *stream);
-int tsk_encode_pack(const tsk_encode_args *args, void *stream);
-int tsk_encode_finish(const tsk_encode_args *args, void *stream);
+/* Clears both look-back buffers; once before the first window of an execution
+ * (each window clears the buffer the next one uses). */
+int tsk_encode_reset(const tsk_encode_args *args, void *stream);
+/* Encodes one window: index entries, payloads, and for each image that ends
+ * in it the header and result.  Windows of one execution run in order. */
+int tsk_encode_window(const tsk_encode_args *args, void *stream);
 int tsk_calibrate(const tsk_calibrate_args *args, void *stream);
 /* Loads every kernel of an operation so that no execution loads code.
  * op: 0 compress, 1 decode, 2 calibrate, 3 fused decode. */

@@ -1,14 +1,33 @@
-// Canonical GPU encoder for gfx950: the same bytes as the CPU encoder.
+// Canonical GPU encoder for gfx950: the same bytes as the CPU encoder, in one
+// pass over the input.
 //
-// Per window of at most 4096 tiles (reference src/cuda/encoder.cu:290-327):
-//   analyze  one workgroup per tile: residuals, per-row minimal widths, the
-//            three candidate costs and the mode with tie order RAW < CONSTANT <
-//            PREDICT_BITPACK (docs/format.md "Canonical encoder").
-//   scan     one workgroup: prefix of payload lengths over the window.
-//   pack     one workgroup per tile: index entry and payload; every output word
-//            has a single writer that gathers the residuals overlapping it.
-//   finish   per image: the 64-byte header and the result record, or the output
-//            cursor carried into the next window.
+// One launch per window of tiles (reference src/cuda/encoder.cu:290-327 runs
+// analyze, a device scan, pack and finish per window; here they are one
+// kernel).  Workgroup b encodes window tile b.  Each XCD dispatches its
+// workgroups in increasing order, so the smallest unfinished tile is always
+// running and waits only on finished tiles.  A workgroup:
+//   stage    the tile's samples into LDS (16-byte loads when rows are aligned);
+//   analyze  residuals, per-row minimal widths, the three candidate costs and
+//            the mode with tie order RAW < CONSTANT < PREDICT_BITPACK
+//            (reference src/cuda/encoder.cu:62-138, docs/format.md
+//            "Canonical encoder"), with lane-group reductions per row;
+//   place    publishes its payload length, then finds its payload offset by a
+//            decoupled look-back over the tiles before it, restarting at
+//            the first tile of its image (a segmented scan; reference
+//            src/cuda/encoder.cu:284-288 uses a device-wide scan instead);
+//   pack     builds the payload in LDS while the offset is still unknown
+//            (each lane ORs its residuals, held in registers since the
+//            analysis, into the row bit streams), then writes the index entry
+//            and the payload (reference src/cuda/encoder.cu:175-238); the
+//            last tile of an image writes
+//            the 64-byte header and the result record (reference
+//            src/cuda/encoder.cu:240-280).
+// The look-back waits are bounded: a workgroup that waits too long marks its
+// published prefix as failed, the mark travels with every prefix and carried
+// cursor built on it, and the image's result reports hip_error instead of a
+// hang.  Two look-back state buffers alternate by window parity: a window's
+// workgroups clear the other buffer for the next window, so no workgroup
+// needs a device-wide fence (which writes back the L2 of its XCD).
 #include <hip/hip_runtime.h>
 
 #include <tilestream_kernels.h>
@@ -18,6 +37,14 @@
 namespace {
 
 constexpr std::uint32_t kThreads = 256;
+constexpr std::uint32_t kMaxTileWords = 4096 + 64 * 4; // rows of tile_w + lanes-per-row words
+constexpr std::uint64_t kFlagAggregate = 1ull << 62;    // value: this tile's length
+constexpr std::uint64_t kFlagPrefix = 2ull << 62;       // value: lengths from the image's first tile here through this one
+constexpr std::uint64_t kFlagMask = 3ull << 62;
+constexpr std::uint64_t kFailed = 1ull << 61;           // a wait before this value timed out
+constexpr std::uint64_t kValueMask = kFailed - 1;
+constexpr std::uint32_t kSpinLimit = 1u << 20;
+constexpr std::uint32_t kHipError = 9;
 
 struct DeviceResultBits {
     std::uint32_t status;
@@ -40,14 +67,14 @@ __device__ inline Geometry geometry(const tsk_encode_image &img, std::uint64_t t
 }
 
 // Window entries are ordered by their first tile.
-__device__ inline const tsk_window_entry &owner(const tsk_window_entry *e, std::uint32_t n, std::uint32_t tile) {
+__device__ inline std::uint32_t owner(const tsk_window_entry *e, std::uint32_t n, std::uint32_t tile) {
     std::uint32_t lo = 0, hi = n;
     while (hi - lo > 1) {
         const std::uint32_t mid = lo + (hi - lo) / 2;
         if (e[mid].first <= tile) lo = mid;
         else hi = mid;
     }
-    return e[lo];
+    return lo;
 }
 
 __device__ inline std::uint64_t payload_start(const tsk_encode_image &img) { return 64 + 32 * img.tile_count; }
@@ -61,137 +88,223 @@ template <bool U16> __device__ inline std::uint32_t residual(std::uint32_t cur, 
     }
 }
 
-// Loads the tile's samples row-major (w wide) into LDS.
+// floor(p / d) as a multiply-high by ceil(2^32 / d): exact while p * d < 2^32
+// (p below 2^17 and d at most 256 here).
+__device__ inline std::uint32_t magic_of(std::uint32_t d) { return static_cast<std::uint32_t>(0xffffffffu / d) + 1u; }
+__device__ inline std::uint32_t div_magic(std::uint32_t p, std::uint32_t magic, std::uint32_t d) {
+    return d == 1u ? p : __umulhi(p, magic);
+}
+
+__device__ inline std::uint64_t load_state(const std::uint64_t *p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline void store_state(std::uint64_t *p, std::uint64_t v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ inline std::uint64_t wave_sum(std::uint64_t v) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+    return v;
+}
+
+// Copies the tile into LDS, row r at px + r * stride, one 32-bit word per sample.
 template <bool U16>
-__device__ inline void load_tile(const tsk_encode_image &img, const Geometry &g, std::uint32_t *px) {
-    const std::uint32_t n = g.w * g.h;
-    const std::uint8_t *origin = img.in + g.y * img.in_pitch + g.x * (U16 ? 2 : 4);
-    for (std::uint32_t k = threadIdx.x; k < n; k += kThreads) {
-        const std::uint32_t r = k / g.w, c = k - r * g.w;
-        const std::uint8_t *p = origin + std::uint64_t{r} * img.in_pitch;
-        px[k] = U16 ? reinterpret_cast<const std::uint16_t *>(p)[c] : reinterpret_cast<const std::uint32_t *>(p)[c];
-    }
-}
-
-__device__ inline std::uint32_t row_cost(std::uint32_t w, std::uint32_t b) { return 8u + (((w - 1u) * b + 31u) / 32u) * 4u; }
-
-template <bool U16> __global__ void __launch_bounds__(kThreads) analyze_kernel(tsk_encode_args a) {
-    __shared__ std::uint32_t px[4096];
-    __shared__ std::uint32_t row_or[64];
-
-    const std::uint32_t t = blockIdx.x;
-    const tsk_window_entry &e = owner(a.entries, a.entry_count, t);
-    const tsk_encode_image &img = a.images[e.image];
-    const Geometry g = geometry(img, e.start + (t - e.first));
-    if (threadIdx.x < 64) row_or[threadIdx.x] = 0;
-    load_tile<U16>(img, g, px);
-    __syncthreads();
-
-    const std::uint32_t n = g.w * g.h;
-    const std::uint32_t first = px[0];
-    bool same = true;
-    for (std::uint32_t k = threadIdx.x; k < n; k += kThreads) {
-        const std::uint32_t r = k / g.w, c = k - r * g.w;
-        same = same && px[k] == first;
-        if (c != 0) {
-            const std::uint32_t z = residual<U16>(px[k], px[k - 1]);
-            if (z != 0) atomicOr(&row_or[r], z);
-        }
-    }
-    const bool uniform = __syncthreads_and(same) != 0;
-
-    // Row widths and the predictive cost, reduced over the first wave (h <= 64).
-    std::uint32_t cost = 0;
-    if (threadIdx.x < g.h) {
-        const std::uint32_t any = row_or[threadIdx.x];
-        const std::uint32_t b = any == 0 ? 0u : 32u - static_cast<std::uint32_t>(__builtin_clz(any));
-        a.rows[std::uint64_t{t} * a.max_rows + threadIdx.x] = static_cast<std::uint8_t>(b);
-        cost = row_cost(g.w, b);
-    }
-    if (threadIdx.x < 64) {
-        for (int off = 32; off > 0; off >>= 1) cost += __shfl_xor(cost, off);
-        if (threadIdx.x == 0) {
-            const std::uint32_t raw4 = (g.w * g.h * (U16 ? 2u : 4u) + 3u) / 4u * 4u;
-            std::uint32_t mode = 0, length = raw4;
-            if (uniform && length > 4u) {
-                mode = 1;
-                length = 4;
+__device__ inline void stage_tile(const tsk_encode_image &img, const Geometry &g, std::uint32_t stride, std::uint32_t *px) {
+    constexpr std::uint32_t s = U16 ? 2u : 4u;
+    const std::uint8_t *origin = img.in + g.y * img.in_pitch + g.x * s;
+    const bool vector = ((reinterpret_cast<std::uintptr_t>(img.in) | img.in_pitch) & 15u) == 0;
+    if (vector) {
+        // Rows start 16-byte aligned (tile widths are multiples of 64 samples).
+        constexpr std::uint32_t per = 16u / s;
+        const std::uint32_t across = (g.w + per - 1u) / per;
+        const std::uint32_t chunks = across * g.h;
+        const std::uint32_t magic = magic_of(across);
+        for (std::uint32_t k = threadIdx.x; k < chunks; k += kThreads) {
+            const std::uint32_t r = div_magic(k, magic, across), c = k - r * across;
+            const std::uint8_t *p = origin + std::uint64_t{r} * img.in_pitch + c * 16u;
+            std::uint32_t *dst = px + r * stride + c * per;
+            if ((c + 1u) * per <= g.w) {
+                const uint4 v = *reinterpret_cast<const uint4 *>(p);
+                if constexpr (U16) {
+                    reinterpret_cast<uint4 *>(dst)[0] = make_uint4(v.x & 0xffffu, v.x >> 16, v.y & 0xffffu, v.y >> 16);
+                    reinterpret_cast<uint4 *>(dst)[1] = make_uint4(v.z & 0xffffu, v.z >> 16, v.w & 0xffffu, v.w >> 16);
+                } else {
+                    reinterpret_cast<uint4 *>(dst)[0] = v;
+                }
+            } else {
+                // The image's right edge: only the samples inside it.
+                for (std::uint32_t q = 0; c * per + q < g.w; ++q)
+                    dst[q] = U16 ? reinterpret_cast<const std::uint16_t *>(p)[q] : reinterpret_cast<const std::uint32_t *>(p)[q];
             }
-            if (cost < length) {
-                mode = 2;
-                length = cost;
-            }
-            a.tiles[2 * t] = mode;
-            a.tiles[2 * t + 1] = length;
+        }
+    } else {
+        const std::uint32_t n = g.w * g.h;
+        const std::uint32_t magic = magic_of(g.w);
+        for (std::uint32_t k = threadIdx.x; k < n; k += kThreads) {
+            const std::uint32_t r = div_magic(k, magic, g.w), c = k - r * g.w;
+            const std::uint8_t *p = origin + std::uint64_t{r} * img.in_pitch;
+            px[r * stride + c] = U16 ? reinterpret_cast<const std::uint16_t *>(p)[c] : reinterpret_cast<const std::uint32_t *>(p)[c];
         }
     }
 }
 
-// offsets[0] = 0, offsets[t + 1] = sum of lengths of window tiles 0..t.
-__global__ void __launch_bounds__(1024) scan_kernel(tsk_encode_args a) {
-    __shared__ std::uint64_t wave_sums[16];
-    const std::uint32_t n = a.tile_count;
-    const std::uint32_t per = (n + 1023) / 1024;
-    const std::uint32_t begin = threadIdx.x * per;
-    std::uint64_t local = 0;
-    for (std::uint32_t i = begin; i < begin + per && i < n; ++i) local += a.tiles[2 * i + 1];
-    // Inclusive scan of the thread totals: within each wave, then over waves.
-    const std::uint32_t lane = threadIdx.x % 64, wave = threadIdx.x / 64;
-    std::uint64_t incl = local;
-    for (std::uint32_t off = 1; off < 64; off <<= 1) {
-        const std::uint64_t other = __shfl_up(incl, off);
-        if (lane >= off) incl += other;
-    }
-    if (lane == 63) wave_sums[wave] = incl;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        std::uint64_t run = 0;
-        for (std::uint32_t w = 0; w < 16; ++w) {
-            const std::uint64_t s = wave_sums[w];
-            wave_sums[w] = run;
-            run += s;
-        }
-    }
-    __syncthreads();
-    std::uint64_t running = wave_sums[wave] + incl - local;
-    if (threadIdx.x == 0) a.offsets[0] = 0;
-    for (std::uint32_t i = begin; i < begin + per && i < n; ++i) {
-        running += a.tiles[2 * i + 1];
-        a.offsets[i + 1] = running;
-    }
-}
-
-// Word `m` of a row's packed residuals: the bits [32 m, 32 m + 32) of the
-// LSB-first stream of w - 1 residuals of b bits.
-template <bool U16>
-__device__ inline std::uint32_t packed_word(const std::uint32_t *row, std::uint32_t w, std::uint32_t b, std::uint32_t m) {
-    const std::uint32_t lo_bit = 32u * m;
-    const std::uint32_t last = w - 2u; // last residual index
-    std::uint32_t j = lo_bit / b;
-    const std::uint32_t jmax = (lo_bit + 31u) / b < last ? (lo_bit + 31u) / b : last;
-    std::uint32_t value = 0;
-    for (; j <= jmax; ++j) {
-        const std::uint32_t z = residual<U16>(row[j + 1], row[j]);
-        const std::uint32_t pos = j * b;
-        value |= pos >= lo_bit ? z << (pos - lo_bit) : z >> (lo_bit - pos);
-    }
-    return value;
-}
-
-template <bool U16> __global__ void __launch_bounds__(kThreads) pack_kernel(tsk_encode_args a) {
-    __shared__ std::uint32_t px[4096];
+template <bool U16> __global__ void __launch_bounds__(kThreads) encode_kernel(tsk_encode_args a) {
+    __shared__ __attribute__((aligned(16))) std::uint32_t px[kMaxTileWords];
     __shared__ std::uint32_t row_word[65];
     __shared__ std::uint32_t row_bits[64];
+    __shared__ std::uint32_t wave_cost[kThreads / 64];
+    __shared__ std::uint64_t shared_offset;
 
+    const std::uint32_t lane = threadIdx.x % 64u;
     const std::uint32_t t = blockIdx.x;
-    const tsk_window_entry &e = owner(a.entries, a.entry_count, t);
+    std::uint64_t *state = a.state + std::uint64_t{a.parity} * a.capacity;
+    {
+        // Clear the other buffer for the next window (the previous one is done with it).
+        std::uint64_t *other = a.state + std::uint64_t{a.parity ^ 1u} * a.capacity;
+        for (std::uint32_t k = t + threadIdx.x * a.tile_count; k < a.capacity; k += kThreads * a.tile_count) other[k] = 0;
+    }
+    const tsk_window_entry e = a.entries[owner(a.entries, a.entry_count, t)];
     const tsk_encode_image &img = a.images[e.image];
     const std::uint64_t index = e.start + (t - e.first);
     const Geometry g = geometry(img, index);
-    const std::uint32_t mode = a.tiles[2 * t], length = a.tiles[2 * t + 1];
-    const std::uint64_t base = (e.flags & 1u) ? a.carry[a.parity] : payload_start(img);
-    const std::uint64_t offset = base + a.offsets[t] - a.offsets[e.first];
+    // Lanes per tile row: 4, 8 or 16 for 64, 32 or 16 rows; each lane holds
+    // 16 columns (tile_w / lanes) of its row.  Row r sits at px + r * stride,
+    // with stride = tile_w + lanes so that a wave's reads fall in distinct banks.
+    const std::uint32_t lanes = kThreads / img.tile_h;
+    const std::uint32_t stride = img.tile_w + lanes;
+    const std::uint32_t r = threadIdx.x / lanes, sl = threadIdx.x % lanes;
 
+    stage_tile<U16>(img, g, stride, px);
+    __syncthreads();
+
+    // ---- analyze -----------------------------------------------------------------
+    const std::uint32_t first = px[0];
+    bool same = true;
+    std::uint32_t any = 0, seed = 0;
+    std::uint32_t z[16];
+#pragma unroll
+    for (std::uint32_t i = 0; i < 16; ++i) {
+        const std::uint32_t c = sl + lanes * i;
+        z[i] = 0;
+        if (r < g.h && c < g.w) {
+            const std::uint32_t v = px[r * stride + c];
+            same = same && v == first;
+            if (c == 0) seed = v;
+            if (c != 0) {
+                z[i] = residual<U16>(v, px[r * stride + c - 1u]);
+                any |= z[i];
+            }
+        }
+    }
+    for (std::uint32_t off = lanes / 2u; off > 0; off >>= 1) any |= __shfl_xor(any, static_cast<int>(off));
+    std::uint32_t cost = 0;
+    if (sl == 0 && r < g.h) {
+        const std::uint32_t b = any == 0 ? 0u : 32u - static_cast<std::uint32_t>(__builtin_clz(any));
+        row_bits[r] = b;
+        cost = 8u + (((g.w - 1u) * b + 31u) / 32u) * 4u;
+    }
+    for (int off = 32; off > 0; off >>= 1) cost += __shfl_xor(cost, off);
+    if (lane == 0) wave_cost[threadIdx.x / 64u] = cost;
+    const bool uniform = __syncthreads_and(same) != 0;
+    std::uint32_t total = 0;
+#pragma unroll
+    for (std::uint32_t w = 0; w < kThreads / 64u; ++w) total += wave_cost[w];
+    const std::uint32_t raw4 = (g.w * g.h * (U16 ? 2u : 4u) + 3u) / 4u * 4u;
+    std::uint32_t mode = 0, length = raw4;
+    if (uniform && length > 4u) {
+        mode = 1;
+        length = 4;
+    }
+    if (total < length) {
+        mode = 2;
+        length = total;
+    }
+
+    // ---- place: publish the length, then look back for the offset ----------------
+    const bool segment_start = t == e.first;
+    // A continued image starts at the cursor the previous window left.
+    const std::uint64_t carried = (e.flags & 1u) ? a.carry[a.parity] : payload_start(img);
+    if (threadIdx.x == 0)
+        store_state(&state[t], segment_start ? (kFlagPrefix | (carried & kFailed) | length) : (kFlagAggregate | length));
+    if (mode == 2) {
+        // The payload is built in LDS before the offset is known.  Words of each
+        // row: seed, width, packed residuals; row_word is their exclusive prefix.
+        if (threadIdx.x < 64) {
+            const std::uint32_t words = lane < g.h ? 2u + ((g.w - 1u) * row_bits[lane] + 31u) / 32u : 0u;
+            std::uint32_t incl = words;
+            for (std::uint32_t off = 1; off < 64; off <<= 1) {
+                const std::uint32_t other = __shfl_up(incl, off);
+                if (lane >= off) incl += other;
+            }
+            if (lane == 0) row_word[0] = 0;
+            row_word[lane + 1] = incl;
+        }
+        // The samples were last read before the barrier above: their area
+        // becomes the payload, zeroed first.
+        for (std::uint32_t k = threadIdx.x * 4u; k < length / 4u; k += kThreads * 4u)
+            *reinterpret_cast<uint4 *>(px + k) = make_uint4(0u, 0u, 0u, 0u);
+        __syncthreads();
+        if (r < g.h) {
+            // Each lane ORs its residuals (still in registers) into the row's
+            // LSB-first bit stream: residual of column c at bit (c - 1) * b.
+            const std::uint32_t b = row_bits[r], base = row_word[r];
+            if (sl == 0) {
+                px[base] = seed;
+                px[base + 1u] = b;
+            }
+#pragma unroll
+            for (std::uint32_t i = 0; i < 16; ++i) {
+                const std::uint32_t c = sl + lanes * i;
+                if (z[i] != 0 && c < g.w) {
+                    const std::uint32_t pos = (c - 1u) * b;
+                    const std::uint32_t w = base + 2u + (pos >> 5), sh = pos & 31u;
+                    atomicOr(&px[w], z[i] << sh);
+                    if (sh + b > 32u) atomicOr(&px[w + 1u], z[i] >> (32u - sh));
+                }
+            }
+        }
+    }
+    if (threadIdx.x < 64) {
+        std::uint64_t exclusive = 0;
+        bool failed = (carried & kFailed) != 0;
+        if (!segment_start) {
+            std::int64_t top = static_cast<std::int64_t>(t) - 1;
+            std::uint32_t spins = 0;
+            for (;;) {
+                const std::int64_t j = top - static_cast<std::int64_t>(lane);
+                const std::uint64_t s = j >= static_cast<std::int64_t>(e.first) ? load_state(&state[j]) : 0ull;
+                const std::uint64_t flag = s & kFlagMask;
+                const std::uint64_t prefix_lanes = __ballot(flag == kFlagPrefix);
+                const std::uint64_t empty_lanes = __ballot(flag == 0ull);
+                // The nearest published prefix, and whether every tile before it has published.
+                const std::uint32_t stop = prefix_lanes != 0 ? static_cast<std::uint32_t>(__builtin_ctzll(prefix_lanes)) : 64u;
+                const std::uint64_t needed = stop == 64u ? ~0ull : ((2ull << stop) - 1ull);
+                if ((empty_lanes & needed) == 0) {
+                    const bool mine = lane <= stop;
+                    exclusive += wave_sum(mine ? (s & kValueMask) : 0ull);
+                    failed = failed || __ballot(mine && (s & kFailed) != 0) != 0;
+                    if (stop != 64u) break;
+                    top -= 64;
+                    continue;
+                }
+                if (++spins > kSpinLimit) {
+                    failed = true;
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(2);
+            }
+        }
+        if (lane == 0) {
+            const std::uint64_t mark = failed ? kFailed : 0ull;
+            if (!segment_start) store_state(&state[t], kFlagPrefix | mark | ((exclusive + length) & kValueMask));
+            shared_offset = mark | ((carried & kValueMask) + exclusive);
+        }
+    }
+    __syncthreads();
+    const bool failed = (shared_offset & kFailed) != 0;
+    const std::uint64_t offset = shared_offset & kValueMask;
+
+    // ---- pack ----------------------------------------------------------------------
     if (threadIdx.x < 8) {
         // 32-byte index entry: mode, reserved, width, height, offset, length, reserved.
         std::uint32_t v = 0;
@@ -205,123 +318,97 @@ template <bool U16> __global__ void __launch_bounds__(kThreads) pack_kernel(tsk_
         }
         reinterpret_cast<std::uint32_t *>(img.out + 64 + index * 32)[threadIdx.x] = v;
     }
-    load_tile<U16>(img, g, px);
-    if (mode == 2 && threadIdx.x < g.h) row_bits[threadIdx.x] = a.rows[std::uint64_t{t} * a.max_rows + threadIdx.x];
-    __syncthreads();
-    if (mode == 2 && threadIdx.x == 0) {
-        std::uint32_t cur = 0;
-        for (std::uint32_t r = 0; r < g.h; ++r) {
-            row_word[r] = cur;
-            cur += 2u + ((g.w - 1u) * row_bits[r] + 31u) / 32u;
-        }
-        row_word[g.h] = cur;
-    }
-    __syncthreads();
-
     std::uint32_t *dst = reinterpret_cast<std::uint32_t *>(img.out + offset);
     const std::uint32_t words = length / 4u;
-    const std::uint32_t n = g.w * g.h;
-    for (std::uint32_t k = threadIdx.x; k < words; k += kThreads) {
-        std::uint32_t v = 0;
-        if (mode == 0) {
-            if (U16) {
-                v = px[2 * k] & 0xffffu;
-                if (2 * k + 1 < n) v |= px[2 * k + 1] << 16;
+    if (mode == 2) {
+        for (std::uint32_t k = threadIdx.x; k < words; k += kThreads) dst[k] = px[k];
+    } else if (mode == 0) {
+        const std::uint32_t magic = magic_of(g.w);
+        const std::uint32_t n = g.w * g.h;
+        for (std::uint32_t k = threadIdx.x; k < words; k += kThreads) {
+            std::uint32_t v;
+            if constexpr (U16) {
+                const std::uint32_t p0 = 2u * k, r0 = div_magic(p0, magic, g.w);
+                v = px[r0 * stride + (p0 - r0 * g.w)];
+                if (p0 + 1u < n) {
+                    const std::uint32_t r1 = div_magic(p0 + 1u, magic, g.w);
+                    v |= px[r1 * stride + (p0 + 1u - r1 * g.w)] << 16;
+                }
             } else {
-                v = px[k];
+                const std::uint32_t r0 = div_magic(k, magic, g.w);
+                v = px[r0 * stride + (k - r0 * g.w)];
             }
-        } else if (mode == 1) {
-            v = px[0];
-        } else {
-            std::uint32_t lo = 0, hi = g.h;
-            while (hi - lo > 1) {
-                const std::uint32_t mid = (lo + hi) / 2;
-                if (row_word[mid] <= k) lo = mid;
-                else hi = mid;
-            }
-            const std::uint32_t local = k - row_word[lo];
-            if (local == 0) v = px[lo * g.w];
-            else if (local == 1) v = row_bits[lo];
-            else v = packed_word<U16>(px + lo * g.w, g.w, row_bits[lo], local - 2);
+            dst[k] = v;
         }
-        dst[k] = v;
+    } else if (threadIdx.x == 0) {
+        dst[0] = first;
     }
+
+    // ---- the image's end: header and result, or the cursor for the next window ----
+    if (threadIdx.x == 0 && t == e.first + e.count - 1u) {
+        const std::uint64_t end = offset + length;
+        if ((e.flags & 2u) == 0) {
+            a.carry[a.parity ^ 1u] = end | (failed ? kFailed : 0ull);
+        } else if (failed) {
+            static_cast<DeviceResultBits *>(a.results)[e.image] = {kHipError, 0u, 0u};
+        } else {
+            // 64-byte file header (docs/format.md "File header").
+            const std::uint64_t count = img.tile_count;
+            const std::uint64_t payload = payload_start(img);
+            std::uint32_t h[16];
+            h[0] = 0x00435354u; // 'T' 'S' 'C' 0
+            h[1] = 0x0a1a0a0du; // 13 10 26 10
+            h[2] = 1u;          // major 1, minor 0
+            h[3] = 64u | (img.type << 16); // header size, pixel type, flags 0
+            h[4] = img.width;
+            h[5] = img.height;
+            h[6] = img.tile_w | (std::uint32_t{img.tile_h} << 16);
+            h[7] = 32u; // index entry size, reserved 0
+            h[8] = static_cast<std::uint32_t>(count);
+            h[9] = static_cast<std::uint32_t>(count >> 32);
+            h[10] = 64u;
+            h[11] = 0u;
+            h[12] = static_cast<std::uint32_t>(payload);
+            h[13] = static_cast<std::uint32_t>(payload >> 32);
+            h[14] = static_cast<std::uint32_t>(end);
+            h[15] = static_cast<std::uint32_t>(end >> 32);
+            std::uint32_t *out = reinterpret_cast<std::uint32_t *>(img.out);
+            for (int i = 0; i < 16; ++i) out[i] = h[i];
+            static_cast<DeviceResultBits *>(a.results)[e.image] = {0u, 0u, end};
+        }
+    }
+
 }
 
-__global__ void finish_kernel(tsk_encode_args a) {
-    const std::uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= a.entry_count) return;
-    const tsk_window_entry e = a.entries[j];
-    const tsk_encode_image &img = a.images[e.image];
-    const std::uint64_t base = (e.flags & 1u) ? a.carry[a.parity] : payload_start(img);
-    const std::uint64_t end = base + a.offsets[e.first + e.count] - a.offsets[e.first];
-    if ((e.flags & 2u) == 0) {
-        a.carry[a.parity ^ 1u] = end;
-        return;
-    }
-    // 64-byte file header (docs/format.md "File header").
-    const std::uint64_t count = img.tile_count;
-    const std::uint64_t payload = payload_start(img);
-    std::uint32_t h[16];
-    h[0] = 0x00435354u;  // 'T' 'S' 'C' 0
-    h[1] = 0x0a1a0a0du;  // 13 10 26 10
-    h[2] = 1u | (0u << 16);             // major 1, minor 0
-    h[3] = 64u | (img.type << 16);      // header size, pixel type, flags 0
-    h[4] = img.width;
-    h[5] = img.height;
-    h[6] = img.tile_w | (std::uint32_t{img.tile_h} << 16);
-    h[7] = 32u;                          // index entry size, reserved 0
-    h[8] = static_cast<std::uint32_t>(count);
-    h[9] = static_cast<std::uint32_t>(count >> 32);
-    h[10] = 64u;
-    h[11] = 0u;
-    h[12] = static_cast<std::uint32_t>(payload);
-    h[13] = static_cast<std::uint32_t>(payload >> 32);
-    h[14] = static_cast<std::uint32_t>(end);
-    h[15] = static_cast<std::uint32_t>(end >> 32);
-    std::uint32_t *out = reinterpret_cast<std::uint32_t *>(img.out);
-    for (int i = 0; i < 16; ++i) out[i] = h[i];
-    static_cast<DeviceResultBits *>(a.results)[e.image] = {0u, 0u, end};
+__global__ void clear_kernel(std::uint64_t *state, std::uint64_t words) {
+    const std::uint64_t stride = std::uint64_t{gridDim.x} * blockDim.x;
+    for (std::uint64_t k = std::uint64_t{blockIdx.x} * blockDim.x + threadIdx.x; k < words; k += stride) state[k] = 0;
 }
 
 } // namespace
 
 extern "C" {
 
-int tsk_encode_analyze(const tsk_encode_args *a, void *stream) {
+int tsk_encode_reset(const tsk_encode_args *a, void *stream) {
+    const std::uint64_t words = 2ull * a->capacity;
+    const std::uint32_t blocks = static_cast<std::uint32_t>(words / 256u + 1u < 1024u ? words / 256u + 1u : 1024u);
+    hipLaunchKernelGGL(clear_kernel, dim3(blocks), dim3(256), 0, static_cast<hipStream_t>(stream), a->state, words);
+    return hipGetLastError();
+}
+
+int tsk_encode_window(const tsk_encode_args *a, void *stream) {
     if (a->tile_count == 0) return hipSuccess;
     if (a->type == TSK_UINT16)
-        hipLaunchKernelGGL(analyze_kernel<true>, dim3(a->tile_count), dim3(kThreads), 0, static_cast<hipStream_t>(stream), *a);
+        hipLaunchKernelGGL(encode_kernel<true>, dim3(a->tile_count), dim3(kThreads), 0, static_cast<hipStream_t>(stream), *a);
     else
-        hipLaunchKernelGGL(analyze_kernel<false>, dim3(a->tile_count), dim3(kThreads), 0, static_cast<hipStream_t>(stream), *a);
-    return hipGetLastError();
-}
-
-int tsk_encode_scan(const tsk_encode_args *a, void *stream) {
-    hipLaunchKernelGGL(scan_kernel, dim3(1), dim3(1024), 0, static_cast<hipStream_t>(stream), *a);
-    return hipGetLastError();
-}
-
-int tsk_encode_pack(const tsk_encode_args *a, void *stream) {
-    if (a->tile_count == 0) return hipSuccess;
-    if (a->type == TSK_UINT16)
-        hipLaunchKernelGGL(pack_kernel<true>, dim3(a->tile_count), dim3(kThreads), 0, static_cast<hipStream_t>(stream), *a);
-    else
-        hipLaunchKernelGGL(pack_kernel<false>, dim3(a->tile_count), dim3(kThreads), 0, static_cast<hipStream_t>(stream), *a);
-    return hipGetLastError();
-}
-
-int tsk_encode_finish(const tsk_encode_args *a, void *stream) {
-    if (a->entry_count == 0) return hipSuccess;
-    hipLaunchKernelGGL(finish_kernel, dim3((a->entry_count + 255) / 256), dim3(256), 0, static_cast<hipStream_t>(stream), *a);
+        hipLaunchKernelGGL(encode_kernel<false>, dim3(a->tile_count), dim3(kThreads), 0, static_cast<hipStream_t>(stream), *a);
     return hipGetLastError();
 }
 
 int tsk_preload_encode(void) {
     hipFuncAttributes attr;
-    const void *fns[] = {reinterpret_cast<const void *>(analyze_kernel<true>), reinterpret_cast<const void *>(analyze_kernel<false>),
-                         reinterpret_cast<const void *>(scan_kernel),          reinterpret_cast<const void *>(pack_kernel<true>),
-                         reinterpret_cast<const void *>(pack_kernel<false>),   reinterpret_cast<const void *>(finish_kernel)};
+    const void *fns[] = {reinterpret_cast<const void *>(encode_kernel<true>), reinterpret_cast<const void *>(encode_kernel<false>),
+                         reinterpret_cast<const void *>(clear_kernel)};
     for (const void *f : fns) {
         const hipError_t e = hipFuncGetAttributes(&attr, f);
         if (e != hipSuccess) return e;

@@ -659,21 +659,21 @@ Status compress_async(const BatchPlan &plan, MutableBytes workspace, ResultBuffe
     tsk_encode_args a{};
     a.images = state->encode_images;
     a.results = results.data;
-    a.rows = ws + state->layout.rows;
-    a.tiles = reinterpret_cast<std::uint32_t *>(ws + state->layout.tiles);
-    a.offsets = reinterpret_cast<std::uint64_t *>(ws + state->layout.offsets);
+    // Look-back buffers in the residual region (16 KiB per window tile), the
+    // carried cursor in the image region.
+    a.state = reinterpret_cast<std::uint64_t *>(ws + state->layout.residuals);
     a.carry = reinterpret_cast<std::uint64_t *>(ws + state->layout.images);
-    a.max_rows = state->spec.tile.height;
+    a.capacity = static_cast<std::uint32_t>(state->spec.tile_count);
     a.type = static_cast<std::uint32_t>(state->type);
+    if (tsk_encode_reset(&a, stream) != hipSuccess)
+        return detail::hip_failure();
     for (std::size_t k = 0; k < state->windows.size(); ++k) {
         const detail::EncodeWindow &w = state->windows[k];
         a.entries = w.entries;
         a.entry_count = w.entry_count;
         a.tile_count = w.tile_count;
         a.parity = static_cast<std::uint32_t>(k & 1u);
-        if (tsk_encode_analyze(&a, stream) != hipSuccess || tsk_encode_scan(&a, stream) != hipSuccess ||
-            tsk_encode_pack(&a, stream) != hipSuccess || tsk_encode_finish(&a, stream) != hipSuccess)
-            return detail::hip_failure();
+        if (tsk_encode_window(&a, stream) != hipSuccess) return detail::hip_failure();
     }
     return Status::success;
 }

@@ -179,7 +179,7 @@ struct Slot {
     std::uint8_t *host = nullptr;      // tables, then the published results
     std::uint8_t *host_view = nullptr; // the same memory as the device addresses it
     std::uint8_t *device = nullptr;
-    std::uint8_t *scratch = nullptr;   // encoder workspace (rows, tiles, offsets, carry)
+    std::uint8_t *scratch = nullptr;   // encoder workspace (look-back buffers, carried cursor)
     std::uint64_t published = 0;       // offset of the published results in `host`
     std::vector<std::uint64_t> order;  // request index of each result of the current batch
     std::uint64_t bytes = 0;           // output bytes admitted
@@ -187,7 +187,6 @@ struct Slot {
 };
 
 constexpr std::uint64_t kCounterBytes = 256;
-constexpr std::uint32_t kMaxTileRows = 64;
 
 struct Layout {
     std::uint64_t results, images, requests, index, encode, entries, total;
@@ -205,12 +204,12 @@ Layout layout_for(std::uint64_t regions, std::uint64_t encodes, std::uint64_t ta
     return l;
 }
 
-// Encoder workspace per slot: one window of tiles at a time.
+// Encoder workspace per slot: two look-back buffers of one window of tiles
+// (alternating by window parity) and the carried cursor.
+constexpr std::uint32_t kWindowTiles = 1u << 16;
 struct Scratch {
-    static constexpr std::uint64_t rows = 0;
-    static constexpr std::uint64_t tiles = rows + std::uint64_t{detail::window_tiles} * kMaxTileRows;
-    static constexpr std::uint64_t offsets = tiles + std::uint64_t{detail::window_tiles} * 8;
-    static constexpr std::uint64_t carry = offsets + (std::uint64_t{detail::window_tiles} + 1) * 8;
+    static constexpr std::uint64_t state = 0;
+    static constexpr std::uint64_t carry = state + 2 * std::uint64_t{kWindowTiles} * 8;
     static constexpr std::uint64_t total = carry + 256;
 };
 
@@ -223,18 +222,18 @@ struct Extent {
 };
 
 // Splits the tiles of consecutive images into windows of at most
-// window_tiles tiles; returns the entry index where each window starts.
+// kWindowTiles tiles; records the entry index where each window starts.
 void split_windows(const std::vector<std::uint64_t> &tiles, std::vector<tsk_window_entry> &entries,
                    std::vector<std::pair<std::size_t, std::uint32_t>> &windows) {
-    std::uint32_t used = detail::window_tiles;
+    std::uint32_t used = kWindowTiles;
     for (std::size_t image = 0; image < tiles.size(); ++image) {
         for (std::uint64_t start = 0; start < tiles[image];) {
-            if (used == detail::window_tiles) {
+            if (used == kWindowTiles) {
                 windows.push_back({entries.size(), 0});
                 used = 0;
             }
             const auto take = static_cast<std::uint32_t>(
-                std::min<std::uint64_t>(tiles[image] - start, detail::window_tiles - used));
+                std::min<std::uint64_t>(tiles[image] - start, kWindowTiles - used));
             std::uint32_t flags = start == 0 ? 0u : 1u;
             if (start + take == tiles[image]) flags |= 2u;
             entries.push_back({static_cast<std::uint32_t>(image), used, take, flags, start});
@@ -270,7 +269,7 @@ struct Server::Impl {
 
     [[nodiscard]] Layout capacity() const noexcept {
         const std::uint64_t n = options.max_batch;
-        return layout_for(n, n, options.max_tasks, n + 2 * (options.max_tasks / detail::window_tiles + 2));
+        return layout_for(n, n, options.max_tasks, n + 2 * (options.max_tasks / kWindowTiles + 2));
     }
     Status extent(const ServeRequest &r, Extent &e) const noexcept;
     Status launch(Slot &slot, const ServeRequest *requests, std::uint64_t first, std::uint64_t count) noexcept;
@@ -346,7 +345,6 @@ Status Server::Impl::extent(const ServeRequest &r, Extent &e) const noexcept {
         if (r.compressed.data == nullptr || reinterpret_cast<std::uintptr_t>(r.compressed.data) % 4 != 0)
             return Status::invalid_argument;
         if (r.compressed.size < bound) return Status::buffer_too_small;
-        if (r.tile.height > kMaxTileRows) return Status::invalid_argument;
         e.tasks = host::ceil_div(r.input.image.width, r.tile.width) * host::ceil_div(r.input.image.height, r.tile.height);
         if (e.tasks > options.max_tasks) return Status::resource_limit;
         const std::uint64_t s = r.input.image.type == PixelType::uint16 ? 2 : 4;
@@ -516,31 +514,32 @@ Status Server::Impl::launch(Slot &slot, const ServeRequest *requests, std::uint6
                                 slot.host_view + slot.published, slot.stream) != hipSuccess)
             return detail::hip_failure();
     }
-    for (int g = 0; g < 2; ++g) {
-        if (encodes[g] == 0) continue;
+    if (encodes[0] + encodes[1] != 0) {
         tsk_encode_args a{};
-        a.images = reinterpret_cast<const tsk_encode_image *>(slot.device + l.encode) + (g == 0 ? 0 : encodes[0]);
-        a.results = slot.device + l.results + (regions + (g == 0 ? 0 : encodes[0])) * sizeof(DeviceResult);
-        a.rows = slot.scratch + Scratch::rows;
-        a.tiles = reinterpret_cast<std::uint32_t *>(slot.scratch + Scratch::tiles);
-        a.offsets = reinterpret_cast<std::uint64_t *>(slot.scratch + Scratch::offsets);
+        a.state = reinterpret_cast<std::uint64_t *>(slot.scratch + Scratch::state);
         a.carry = reinterpret_cast<std::uint64_t *>(slot.scratch + Scratch::carry);
-        a.max_rows = kMaxTileRows;
-        a.type = g == 0 ? TSK_UINT16 : TSK_FLOAT32;
-        const std::size_t base = g == 0 ? 0 : group1_entries;
-        const std::size_t end_entries = g == 0 ? group1_entries : entries.size();
-        for (std::size_t w = 0; w < windows[g].size(); ++w) {
-            const std::size_t e0 = windows[g][w].first;
-            const std::size_t e1 = w + 1 < windows[g].size() ? windows[g][w + 1].first : end_entries;
-            a.entries = reinterpret_cast<const tsk_window_entry *>(slot.device + l.entries) + e0;
-            a.entry_count = static_cast<std::uint32_t>(e1 - e0);
-            a.tile_count = windows[g][w].second;
-            a.parity = static_cast<std::uint32_t>(w & 1u);
-            if (tsk_encode_analyze(&a, slot.stream) != hipSuccess || tsk_encode_scan(&a, slot.stream) != hipSuccess ||
-                tsk_encode_pack(&a, slot.stream) != hipSuccess || tsk_encode_finish(&a, slot.stream) != hipSuccess)
-                return detail::hip_failure();
+        a.capacity = kWindowTiles;
+        if (tsk_encode_reset(&a, slot.stream) != hipSuccess) return detail::hip_failure();
+        // Window parity alternates over the whole sequence (both pixel types):
+        // each window's look-back buffer was cleared by the window before it,
+        // or by the reset for the first.
+        std::uint32_t sequence = 0;
+        for (int g = 0; g < 2; ++g) {
+            if (encodes[g] == 0) continue;
+            a.images = reinterpret_cast<const tsk_encode_image *>(slot.device + l.encode) + (g == 0 ? 0 : encodes[0]);
+            a.results = slot.device + l.results + (regions + (g == 0 ? 0 : encodes[0])) * sizeof(DeviceResult);
+            a.type = g == 0 ? TSK_UINT16 : TSK_FLOAT32;
+            const std::size_t end_entries = g == 0 ? group1_entries : entries.size();
+            for (std::size_t w = 0; w < windows[g].size(); ++w) {
+                const std::size_t e0 = windows[g][w].first;
+                const std::size_t e1 = w + 1 < windows[g].size() ? windows[g][w + 1].first : end_entries;
+                a.entries = reinterpret_cast<const tsk_window_entry *>(slot.device + l.entries) + e0;
+                a.entry_count = static_cast<std::uint32_t>(e1 - e0);
+                a.tile_count = windows[g][w].second;
+                a.parity = sequence++ & 1u;
+                if (tsk_encode_window(&a, slot.stream) != hipSuccess) return detail::hip_failure();
+            }
         }
-        static_cast<void>(base);
     }
     if (encodes[0] + encodes[1] != 0 &&
         tsk_copy_blocks(slot.host_view + slot.published + regions * sizeof(DeviceResult),
