@@ -79,6 +79,7 @@ struct ServerOptions {
     std::uint32_t streams = 3;           // batches in flight at once
     std::uint32_t max_batch = 1024;      // requests per batch
     std::uint32_t max_tasks = 1u << 22;  // tiles per batch (a larger request is rejected)
+    std::uint32_t encode_window = 1u << 20; // tiles per encoder launch, 64 to 2^20 (look-back state: 16 B per tile)
     std::uint64_t admission_bytes = 200ull << 30; // output bytes in flight (encode: its compress bound)
     LaunchShape shape;                   // decode kernel launch shape (grid 0: sized per batch)
 };

@@ -165,7 +165,7 @@ typedef struct {
     float offset, gain;
 } ts_serve_request;
 typedef struct {
-    uint32_t streams, max_batch, max_tasks, reserved;
+    uint32_t streams, max_batch, max_tasks, encode_window;
     uint64_t admission_bytes;
     ts_launch_shape shape;
 } ts_server_options;

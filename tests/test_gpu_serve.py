@@ -330,3 +330,29 @@ def test_encode_request_checks(resident):
     assert st == ts.Status.invalid_argument
     st, *_ = server.run_batch([ok, ok])
     assert st == ts.Status.invalid_argument
+
+
+@pytest.mark.parametrize("window", [64, 200])
+def test_encode_windows_carry_cursors(orc, resident, window):
+    """Small encoder windows: images split across windows, middle windows that
+    both continue and are continued, parity alternating across pixel types."""
+    jobs = EncodeJobs(ENCODES, 430)
+    rng = np.random.default_rng(431)
+    specs = _requests(resident, rng, 10)
+    out = Outputs(resident, specs)
+    requests = jobs.requests[:3] + list(out.requests) + jobs.requests[3:]
+    server = api.Server(resident.set, api.server_options(streams=2, max_batch=64, encode_window=window))
+    st, statuses, written = server.run_batch(requests)
+    assert st == ts.Status.success and (statuses == 0).all()
+    where = [0, 1, 2] + [3 + len(specs) + k for k in range(len(ENCODES) - 3)]
+    for k, at in enumerate(where):
+        want = orc.compress(jobs.pixels[k], ENCODES[k][3])
+        assert jobs.file(k, int(written[at])) == want, (window, k, ENCODES[k])
+    for i, spec in enumerate(specs):
+        assert _same(out.read(specs, i), _expected(orc, resident, spec)), (i, spec)
+
+
+def test_server_rejects_bad_encode_window(resident):
+    for window in (0, 63, (1 << 20) + 1):
+        with pytest.raises(RuntimeError):
+            api.Server(resident.set, api.server_options(encode_window=window))

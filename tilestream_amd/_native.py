@@ -141,7 +141,7 @@ class ServeRequest(C.Structure):
 
 class ServerOptions(C.Structure):
     _fields_ = [("streams", C.c_uint32), ("max_batch", C.c_uint32), ("max_tasks", C.c_uint32),
-                ("reserved", C.c_uint32), ("admission_bytes", C.c_uint64), ("shape", LaunchShape)]
+                ("encode_window", C.c_uint32), ("admission_bytes", C.c_uint64), ("shape", LaunchShape)]
 
 
 class ServerStats(C.Structure):

@@ -489,6 +489,22 @@ Status build(const BatchRequest &request, PlanState &plan, hipStream_t stream) {
         if ((st = accessible(p, plan.device, plan.pageable)) != Status::success) return st;
 
     WindowPlan windows = split_windows(d.tiles);
+    // One window over the whole batch when its look-back buffers (2 x 8 bytes
+    // per tile) fit the residual region the spec reserves (16 KiB per window tile).
+    std::uint64_t total_tiles = 0;
+    for (std::uint64_t n : d.tiles) total_tiles += n;
+    const std::size_t whole_at = windows.entries.size();
+    const bool whole = request.operation == Operation::compress && windows.windows.size() > 1 &&
+                       total_tiles <= std::uint64_t{INT32_MAX} &&
+                       2 * 8 * total_tiles <= std::uint64_t{window_tiles} * 4096 * 4;
+    if (whole) {
+        std::uint64_t first = 0;
+        for (std::size_t image = 0; image < d.tiles.size(); ++image) {
+            windows.entries.push_back({static_cast<std::uint32_t>(image), static_cast<std::uint32_t>(first),
+                                       static_cast<std::uint32_t>(d.tiles[image]), 2u, 0u});
+            first += d.tiles[image];
+        }
+    }
     plan.spec = {0, 0, d.shape, 0};
     for (const auto &[first, w] : windows.windows) {
         plan.spec.image_count = std::max<std::uint64_t>(plan.spec.image_count, w.entry_count);
@@ -527,6 +543,11 @@ Status build(const BatchRequest &request, PlanState &plan, hipStream_t stream) {
         EncodeWindow placed = w;
         placed.entries = reinterpret_cast<const tsk_window_entry *>(base + entries_at) + first;
         plan.windows.push_back(placed);
+    }
+    if (whole) {
+        plan.whole.entries = reinterpret_cast<const tsk_window_entry *>(base + entries_at) + whole_at;
+        plan.whole.entry_count = static_cast<std::uint32_t>(d.tiles.size());
+        plan.whole.tile_count = static_cast<std::uint32_t>(total_tiles);
     }
     const std::uint32_t op = request.operation == Operation::compress    ? 0u
                              : request.operation == Operation::calibrate ? 2u
@@ -665,6 +686,16 @@ Status compress_async(const BatchPlan &plan, MutableBytes workspace, ResultBuffe
     a.carry = reinterpret_cast<std::uint64_t *>(ws + state->layout.images);
     a.capacity = static_cast<std::uint32_t>(state->spec.tile_count);
     a.type = static_cast<std::uint32_t>(state->type);
+    if (state->whole.tile_count != 0) {
+        a.capacity = state->whole.tile_count;
+        a.entries = state->whole.entries;
+        a.entry_count = state->whole.entry_count;
+        a.tile_count = state->whole.tile_count;
+        a.parity = 0;
+        if (tsk_encode_reset(&a, stream) != hipSuccess || tsk_encode_window(&a, stream) != hipSuccess)
+            return detail::hip_failure();
+        return Status::success;
+    }
     if (tsk_encode_reset(&a, stream) != hipSuccess)
         return detail::hip_failure();
     for (std::size_t k = 0; k < state->windows.size(); ++k) {

@@ -72,6 +72,9 @@ struct PlanState {
     // encode
     const tsk_encode_image *encode_images = nullptr;
     std::vector<EncodeWindow> windows;
+    // All images as one window, used when the workspace's residual region
+    // holds both look-back buffers for every tile of the batch.
+    EncodeWindow whole;
 
     // standalone calibration
     const tsk_calibrate_image *calibrate_images = nullptr;

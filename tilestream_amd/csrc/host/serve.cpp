@@ -204,13 +204,12 @@ Layout layout_for(std::uint64_t regions, std::uint64_t encodes, std::uint64_t ta
     return l;
 }
 
-// Encoder workspace per slot: two look-back buffers of one window of tiles
-// (alternating by window parity) and the carried cursor.
-constexpr std::uint32_t kWindowTiles = 1u << 16;
+// Encoder workspace per slot: the carried cursor, then two look-back buffers
+// of one window of tiles (alternating by window parity).
 struct Scratch {
-    static constexpr std::uint64_t state = 0;
-    static constexpr std::uint64_t carry = state + 2 * std::uint64_t{kWindowTiles} * 8;
-    static constexpr std::uint64_t total = carry + 256;
+    static constexpr std::uint64_t carry = 0;
+    static constexpr std::uint64_t state = 256;
+    static std::uint64_t total(std::uint32_t window) { return state + 2 * std::uint64_t{window} * 8; }
 };
 
 // What a request costs a batch.
@@ -221,19 +220,19 @@ struct Extent {
     std::uint64_t decoded = 0; // region pixels x stored sample bytes, or encoded input bytes
 };
 
-// Splits the tiles of consecutive images into windows of at most
-// kWindowTiles tiles; records the entry index where each window starts.
-void split_windows(const std::vector<std::uint64_t> &tiles, std::vector<tsk_window_entry> &entries,
+// Splits the tiles of consecutive images into windows of at most `window`
+// tiles; records the entry index where each window starts.
+void split_windows(const std::vector<std::uint64_t> &tiles, std::uint32_t window, std::vector<tsk_window_entry> &entries,
                    std::vector<std::pair<std::size_t, std::uint32_t>> &windows) {
-    std::uint32_t used = kWindowTiles;
+    std::uint32_t used = window;
     for (std::size_t image = 0; image < tiles.size(); ++image) {
         for (std::uint64_t start = 0; start < tiles[image];) {
-            if (used == kWindowTiles) {
+            if (used == window) {
                 windows.push_back({entries.size(), 0});
                 used = 0;
             }
             const auto take = static_cast<std::uint32_t>(
-                std::min<std::uint64_t>(tiles[image] - start, kWindowTiles - used));
+                std::min<std::uint64_t>(tiles[image] - start, window - used));
             std::uint32_t flags = start == 0 ? 0u : 1u;
             if (start + take == tiles[image]) flags |= 2u;
             entries.push_back({static_cast<std::uint32_t>(image), used, take, flags, start});
@@ -269,7 +268,7 @@ struct Server::Impl {
 
     [[nodiscard]] Layout capacity() const noexcept {
         const std::uint64_t n = options.max_batch;
-        return layout_for(n, n, options.max_tasks, n + 2 * (options.max_tasks / kWindowTiles + 2));
+        return layout_for(n, n, options.max_tasks, n + 2 * (options.max_tasks / options.encode_window + 2));
     }
     Status extent(const ServeRequest &r, Extent &e) const noexcept;
     Status launch(Slot &slot, const ServeRequest *requests, std::uint64_t first, std::uint64_t count) noexcept;
@@ -282,7 +281,7 @@ Status Server::create(const ArchiveSet &archives, const ServerOptions &options, 
     const ArchiveSet::Impl *set = archives.impl();
     if (set == nullptr || set->committed == 0 || set->committed != set->archives.size()) return Status::invalid_argument;
     if (options.streams == 0 || options.streams > 16 || options.max_batch == 0 || options.max_tasks == 0 ||
-        options.admission_bytes == 0)
+        options.admission_bytes == 0 || options.encode_window < 64 || options.encode_window > (1u << 20))
         return Status::invalid_argument;
     const LaunchShape &sh = options.shape;
     if (!tsk_decode_supported(sh.group, sh.block, sh.depth)) return Status::invalid_argument;
@@ -322,7 +321,7 @@ Status Server::create(const ArchiveSet &archives, const ServerOptions &options, 
             s.host_view = static_cast<std::uint8_t *>(hv);
             if (hipMalloc(&d, align_up(cap.total, 256)) != hipSuccess) return detail::hip_failure();
             s.device = static_cast<std::uint8_t *>(d);
-            if (hipMalloc(&w, Scratch::total) != hipSuccess) return detail::hip_failure();
+            if (hipMalloc(&w, Scratch::total(options.encode_window)) != hipSuccess) return detail::hip_failure();
             s.scratch = static_cast<std::uint8_t *>(w);
             s.order.reserve(options.max_batch);
         }
@@ -403,9 +402,9 @@ Status Server::Impl::launch(Slot &slot, const ServeRequest *requests, std::uint6
     }
     std::vector<tsk_window_entry> entries;
     std::vector<std::pair<std::size_t, std::uint32_t>> windows[2];
-    split_windows(enc_tiles[0], entries, windows[0]);
+    split_windows(enc_tiles[0], options.encode_window, entries, windows[0]);
     const std::size_t group1_entries = entries.size();
-    split_windows(enc_tiles[1], entries, windows[1]);
+    split_windows(enc_tiles[1], options.encode_window, entries, windows[1]);
     const Layout l = layout_for(regions, encodes[0] + encodes[1], tasks, entries.size());
 
     std::uint8_t *h = slot.host;
@@ -518,7 +517,7 @@ Status Server::Impl::launch(Slot &slot, const ServeRequest *requests, std::uint6
         tsk_encode_args a{};
         a.state = reinterpret_cast<std::uint64_t *>(slot.scratch + Scratch::state);
         a.carry = reinterpret_cast<std::uint64_t *>(slot.scratch + Scratch::carry);
-        a.capacity = kWindowTiles;
+        a.capacity = options.encode_window;
         if (tsk_encode_reset(&a, slot.stream) != hipSuccess) return detail::hip_failure();
         // Window parity alternates over the whole sequence (both pixel types):
         // each window's look-back buffer was cleared by the window before it,
