@@ -46,6 +46,14 @@ constexpr std::uint64_t kValueMask = kFailed - 1;
 constexpr std::uint32_t kSpinLimit = 1u << 20;
 constexpr std::uint32_t kHipError = 9;
 
+// Global memory through address-space-1 pointers: generic (FLAT) accesses
+// count in the LDS wait counter too, so every LDS wait would drain them.
+template <typename T> using gptr = T __attribute__((address_space(1))) *;
+typedef std::uint32_t u32x4 __attribute__((ext_vector_type(4)));
+// Multi-word global stores need only word alignment on CDNA.
+typedef std::uint32_t u32x4w __attribute__((ext_vector_type(4), aligned(4)));
+template <typename T> __device__ inline gptr<T> global(T *p) { return (gptr<T>)p; }
+
 struct DeviceResultBits {
     std::uint32_t status;
     std::uint32_t reserved;
@@ -124,17 +132,18 @@ __device__ inline void stage_tile(const tsk_encode_image &img, const Geometry &g
             const std::uint8_t *p = origin + std::uint64_t{r} * img.in_pitch + c * 16u;
             std::uint32_t *dst = px + r * stride + c * per;
             if ((c + 1u) * per <= g.w) {
-                const uint4 v = *reinterpret_cast<const uint4 *>(p);
+                // Samples are read once: non-temporal loads.
+                const u32x4 v = __builtin_nontemporal_load(reinterpret_cast<gptr<const u32x4>>(global(p)));
                 if constexpr (U16) {
-                    reinterpret_cast<uint4 *>(dst)[0] = make_uint4(v.x & 0xffffu, v.x >> 16, v.y & 0xffffu, v.y >> 16);
-                    reinterpret_cast<uint4 *>(dst)[1] = make_uint4(v.z & 0xffffu, v.z >> 16, v.w & 0xffffu, v.w >> 16);
+                    reinterpret_cast<u32x4 *>(dst)[0] = u32x4{v.x & 0xffffu, v.x >> 16, v.y & 0xffffu, v.y >> 16};
+                    reinterpret_cast<u32x4 *>(dst)[1] = u32x4{v.z & 0xffffu, v.z >> 16, v.w & 0xffffu, v.w >> 16};
                 } else {
-                    reinterpret_cast<uint4 *>(dst)[0] = v;
+                    reinterpret_cast<u32x4 *>(dst)[0] = v;
                 }
             } else {
                 // The image's right edge: only the samples inside it.
                 for (std::uint32_t q = 0; c * per + q < g.w; ++q)
-                    dst[q] = U16 ? reinterpret_cast<const std::uint16_t *>(p)[q] : reinterpret_cast<const std::uint32_t *>(p)[q];
+                    dst[q] = U16 ? reinterpret_cast<gptr<const std::uint16_t>>(global(p))[q] : reinterpret_cast<gptr<const std::uint32_t>>(global(p))[q];
             }
         }
     } else {
@@ -143,7 +152,7 @@ __device__ inline void stage_tile(const tsk_encode_image &img, const Geometry &g
         for (std::uint32_t k = threadIdx.x; k < n; k += kThreads) {
             const std::uint32_t r = div_magic(k, magic, g.w), c = k - r * g.w;
             const std::uint8_t *p = origin + std::uint64_t{r} * img.in_pitch;
-            px[r * stride + c] = U16 ? reinterpret_cast<const std::uint16_t *>(p)[c] : reinterpret_cast<const std::uint32_t *>(p)[c];
+            px[r * stride + c] = U16 ? reinterpret_cast<gptr<const std::uint16_t>>(global(p))[c] : reinterpret_cast<gptr<const std::uint32_t>>(global(p))[c];
         }
     }
 }
@@ -316,12 +325,16 @@ template <bool U16> __global__ void __launch_bounds__(kThreads) encode_kernel(ts
         case 4: v = length; break;
         default: v = 0; break;
         }
-        reinterpret_cast<std::uint32_t *>(img.out + 64 + index * 32)[threadIdx.x] = v;
+        reinterpret_cast<gptr<std::uint32_t>>(global(img.out + 64 + index * 32))[threadIdx.x] = v;
     }
-    std::uint32_t *dst = reinterpret_cast<std::uint32_t *>(img.out + offset);
+    gptr<std::uint32_t> dst = reinterpret_cast<gptr<std::uint32_t>>(global(img.out + offset));
     const std::uint32_t words = length / 4u;
     if (mode == 2) {
-        for (std::uint32_t k = threadIdx.x; k < words; k += kThreads) dst[k] = px[k];
+        // 16-byte stores at the payload's word alignment, then the last words.
+        const std::uint32_t quads = words / 4u;
+        for (std::uint32_t k = threadIdx.x; k < quads; k += kThreads)
+            *reinterpret_cast<gptr<u32x4w>>(dst + 4u * k) = *reinterpret_cast<const u32x4 *>(px + 4u * k);
+        if (threadIdx.x < words - 4u * quads) dst[4u * quads + threadIdx.x] = px[4u * quads + threadIdx.x];
     } else if (mode == 0) {
         const std::uint32_t magic = magic_of(g.w);
         const std::uint32_t n = g.w * g.h;
@@ -372,7 +385,7 @@ template <bool U16> __global__ void __launch_bounds__(kThreads) encode_kernel(ts
             h[13] = static_cast<std::uint32_t>(payload >> 32);
             h[14] = static_cast<std::uint32_t>(end);
             h[15] = static_cast<std::uint32_t>(end >> 32);
-            std::uint32_t *out = reinterpret_cast<std::uint32_t *>(img.out);
+            gptr<std::uint32_t> out = reinterpret_cast<gptr<std::uint32_t>>(global(img.out));
             for (int i = 0; i < 16; ++i) out[i] = h[i];
             static_cast<DeviceResultBits *>(a.results)[e.image] = {0u, 0u, end};
         }
