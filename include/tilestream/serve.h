@@ -2,8 +2,11 @@
 // in the reference, which plans every request on the host with prepare_batch).
 // Requests decode regions of resident archives or encode device images.
 //
-// ArchiveSet keeps archives resident in device memory together with their tile
-// and row tables, planned once when an archive is added.  A batch of region
+// ArchiveSet keeps archives resident in device memory, or in pinned host
+// memory, together with their tile and row tables, planned once when an
+// archive is added.  Tiles of archives in host memory are staged: each batch
+// copies the payloads its requests touch into device memory with a kernel
+// reading over the host link, while other batches decode.  A batch of region
 // requests then uploads only small per-request records; the decode kernel
 // derives each request's tiles on the device.  Server runs continuous
 // batching over several streams: requests that arrive while batches are on
@@ -24,6 +27,8 @@ struct ArchiveInfo {
     TileShape tile;
     std::uint64_t tile_count = 0;
     std::uint64_t file_bytes = 0;
+    std::uint32_t host = 0; // 1: in pinned host memory, staged per batch
+    std::uint32_t reserved = 0;
 };
 
 class ArchiveSet {
@@ -35,8 +40,9 @@ class ArchiveSet {
     [[nodiscard]] static Status create(std::unique_ptr<ArchiveSet> &set) noexcept;
 
     // Adds a complete archive already in device memory of the current device,
-    // validated on the host into `metadata` (parse_metadata of the same bytes).
-    // The bytes must stay allocated and unchanged while the set is in use.
+    // or in mapped pinned host memory (hipHostMalloc), validated on the host
+    // into `metadata` (parse_metadata of the same bytes).  The bytes must stay
+    // allocated and unchanged while the set is in use.
     [[nodiscard]] Status add(ConstBytes device_file, const ImageMetadata &metadata, std::uint32_t &id) noexcept;
     // Uploads the tables of every archive added so far (synchronizes `stream`).
     [[nodiscard]] Status commit(hipStream_t stream) noexcept;
@@ -81,6 +87,7 @@ struct ServerOptions {
     std::uint32_t max_tasks = 1u << 22;  // tiles per batch (a larger request is rejected)
     std::uint32_t encode_window = 1u << 20; // tiles per encoder launch, 64 to 2^20 (look-back state: 16 B per tile)
     std::uint64_t admission_bytes = 200ull << 30; // output bytes in flight (encode: its compress bound)
+    std::uint64_t staging_bytes = 256ull << 20;   // per stream: device staging for tiles of host archives
     LaunchShape shape;                   // decode kernel launch shape (grid 0: sized per batch)
 };
 

@@ -150,10 +150,16 @@ uint32_t ts_queue_autotune(ts_queue *queue, ts_mutable_bytes workspace, ts_devic
 /* ---- extension: serving resident archives (tilestream/serve.h) ---------
  * No counterpart in the reference.  Archives stay in device memory with tile
  * tables planned once; region decodes and encodes are batched continuously
- * over several streams (hip::ServeRequest). */
+ * over several streams (hip::ServeRequest).  Archives may also live in mapped
+ * pinned host memory: their tiles are staged into device memory per batch. */
 typedef struct ts_archive_set ts_archive_set;
 typedef struct ts_server ts_server;
-typedef struct { ts_image_spec image; ts_tile_shape tile; uint64_t tile_count, file_bytes; } ts_archive_info;
+typedef struct {
+    ts_image_spec image;
+    ts_tile_shape tile;
+    uint64_t tile_count, file_bytes;
+    uint32_t host, reserved;       /* host: 1 when the archive is in pinned host memory (tiles staged per batch) */
+} ts_archive_info;
 typedef struct {
     uint32_t kind;                 /* 0 region decode, 1 encode */
     uint32_t archive, calibrated;  /* region; calibrated: FP32 (x - offset) * gain output */
@@ -167,6 +173,7 @@ typedef struct {
 typedef struct {
     uint32_t streams, max_batch, max_tasks, encode_window;
     uint64_t admission_bytes;
+    uint64_t staging_bytes;        /* per stream, device staging for tiles of host archives */
     ts_launch_shape shape;
 } ts_server_options;
 typedef struct { uint64_t requests, batches, decoded_bytes, failed; double seconds; } ts_server_stats;

@@ -26,6 +26,7 @@ extern "C" {
 #endif
 
 #define TSK_NO_ROWS 0xffffffffu
+#define TSK_NOT_STAGED 0xffffffffu
 
 /* Pixel kinds of a decode image / task. */
 enum { TSK_UINT16 = 1, TSK_FLOAT32 = 2 };
@@ -105,6 +106,12 @@ typedef struct {
     uint32_t request_count;      /* request mode when non-zero (requests, request_index set) */
     const tsk_request *requests;
     const uint32_t *request_index; /* task -> request */
+    /* Request mode, archives in host memory: per task, the tile's place in
+     * `staging` in 16-byte blocks (TSK_NOT_STAGED: read the payload where it
+     * is); NULL when no task is staged.  A staged payload keeps its offset
+     * within its 16-byte block. */
+    const uint32_t *staged;
+    const uint8_t *staging;
 } tsk_decode_args;
 
 /* Per-image descriptor of an encode job. */
@@ -167,6 +174,10 @@ int tsk_results_complete(const tsk_decode_image *images, uint32_t count, void *r
                          void *stream);
 /* tsk_results_complete that writes the final records to `out` (for example
  * mapped host memory) and leaves `results` as the decode left them. */
+/* Request mode: copies the 16-byte blocks holding each staged task's payload
+ * from where the resident tile table points (mapped host memory) into
+ * `staging` (args->staged, args->staging set). */
+int tsk_stage_payloads(const tsk_decode_args *args, void *stream);
 int tsk_results_publish(const tsk_decode_image *images, uint32_t count, const void *results, void *out,
                         void *stream);
 int tsk_decode(const tsk_decode_args *args, void *stream);

@@ -356,3 +356,103 @@ def test_server_rejects_bad_encode_window(resident):
     for window in (0, 63, (1 << 20) + 1):
         with pytest.raises(RuntimeError):
             api.Server(resident.set, api.server_options(encode_window=window))
+
+
+class Mixed:
+    """The archives once in device memory and once in pinned host memory (at
+    offsets that are 4 mod 16, so staged payloads keep an odd block phase)."""
+
+    def __init__(self, orc, resident):
+        self.pixels = resident.pixels + resident.pixels
+        self.files = resident.files + resident.files
+        offsets, total = [], 0
+        for data in resident.files:
+            offsets.append(total + 4)
+            total += (len(data) + 4 + 255) // 256 * 256
+        self.host = api.HostMemory(total)
+        for data, off in zip(resident.files, offsets):
+            self.host.upload(np.frombuffer(data, np.uint8), off)
+        self.set = api.ArchiveSet()
+        for i, (data, mem) in enumerate(zip(resident.files, resident.mems)):
+            st, meta = api.parse_metadata(data)
+            st, ident = self.set.add(mem, meta)
+            assert st == ts.Status.success and ident == i
+        for i, (data, off) in enumerate(zip(resident.files, offsets)):
+            st, meta = api.parse_metadata(data)
+            st, ident = self.set.add(self.host, meta, len(data), off)
+            assert st == ts.Status.success and ident == len(ARCHIVES) + i, st
+        assert self.set.commit() == ts.Status.success
+
+
+@pytest.fixture(scope="module")
+def mixed(orc, resident):
+    return Mixed(orc, resident)
+
+
+def _mixed_requests(mixed, rng, count, big=True):
+    specs = []
+    for k in range(count):
+        a = int(rng.integers(0, 2 * len(ARCHIVES)))
+        h, w = mixed.pixels[a].shape
+        if big and k % 5 == 0:
+            rw, rh = int(rng.integers(w // 2, w + 1)), int(rng.integers(h // 2, h + 1))
+        else:
+            rw, rh = int(rng.integers(1, min(w, 150) + 1)), int(rng.integers(1, min(h, 120) + 1))
+        x, y = int(rng.integers(0, w - rw + 1)), int(rng.integers(0, h - rh + 1))
+        specs.append((a, (x, y, rw, rh), bool(rng.integers(0, 2))))
+    return specs
+
+
+def test_host_archives_are_staged(mixed):
+    for i in range(2 * len(ARCHIVES)):
+        st, info = mixed.set.info(i)
+        assert st == ts.Status.success and info.host == (1 if i >= len(ARCHIVES) else 0)
+
+
+@pytest.mark.parametrize("staging,big", [(256 << 20, True), (512 << 10, False)])
+def test_host_and_device_archives_in_one_batch(orc, mixed, staging, big):
+    """Tiles of host archives staged per batch next to tiles read in place;
+    a small staging area splits the batches."""
+    rng = np.random.default_rng(600 + staging % 97)
+    specs = _mixed_requests(mixed, rng, 80, big)
+    out = Outputs(mixed, specs)
+    # With every request arriving at once, one batch takes them all unless the
+    # staging area (small requests only) splits it.
+    server = api.Server(mixed.set, api.server_options(streams=3, max_batch=32 if big else 1024,
+                                                      staging_bytes=staging))
+    st, latency, statuses, _, stats = server.replay(out.requests, np.zeros(len(specs), np.uint64))
+    assert st == ts.Status.success and (statuses == 0).all()
+    if not big:
+        assert stats.batches >= 2
+    for i, spec in enumerate(specs):
+        assert _same(out.read(specs, i), _expected(orc, mixed, spec)), (staging, i, spec)
+
+
+def test_staged_alone_and_in_the_mix_are_identical(mixed):
+    rng = np.random.default_rng(610)
+    specs = _mixed_requests(mixed, rng, 40)
+    mix = Outputs(mixed, specs)
+    server = api.Server(mixed.set, api.server_options(streams=2, max_batch=16))
+    arrivals = np.cumsum(rng.exponential(20_000.0, len(specs))).astype(np.uint64)
+    st, _, statuses, _, _ = server.replay(mix.requests, arrivals)
+    assert st == ts.Status.success and (statuses == 0).all()
+    for i in range(0, len(specs), 3):
+        alone = Outputs(mixed, [specs[i]])
+        st, statuses, _ = server.run_batch(alone.requests)
+        assert st == ts.Status.success and statuses[0] == 0
+        assert np.array_equal(alone.read([specs[i]], 0), mix.read(specs, i)), (i, specs[i])
+
+
+def test_staging_checks(mixed):
+    server = api.Server(mixed.set, api.server_options(staging_bytes=64 << 10))
+    mem = api.DeviceMemory(4 << 20)
+    h, w = mixed.pixels[len(ARCHIVES) + 2].shape
+    whole = api.region_request(len(ARCHIVES) + 2, (0, 0, w, h), api.device_view(mem, ts.PixelType.uint16, w, h))
+    assert server.check(whole) == ts.Status.resource_limit  # its payload exceeds the staging area
+    on_device = api.region_request(2, (0, 0, w, h), api.device_view(mem, ts.PixelType.uint16, w, h))
+    assert server.check(on_device) == ts.Status.success
+    small = api.region_request(len(ARCHIVES) + 2, (0, 0, 64, 16), api.device_view(mem, ts.PixelType.uint16, 64, 16))
+    assert server.check(small) == ts.Status.success
+    for bad in (0, 8):
+        with pytest.raises(RuntimeError):
+            api.Server(mixed.set, api.server_options(staging_bytes=bad))

@@ -130,7 +130,8 @@ class QueueStats(C.Structure):
 
 
 class ArchiveInfo(C.Structure):
-    _fields_ = [("image", ImageSpec), ("tile", TileShape), ("tile_count", C.c_uint64), ("file_bytes", C.c_uint64)]
+    _fields_ = [("image", ImageSpec), ("tile", TileShape), ("tile_count", C.c_uint64), ("file_bytes", C.c_uint64),
+                ("host", C.c_uint32), ("reserved", C.c_uint32)]
 
 
 class ServeRequest(C.Structure):
@@ -141,7 +142,8 @@ class ServeRequest(C.Structure):
 
 class ServerOptions(C.Structure):
     _fields_ = [("streams", C.c_uint32), ("max_batch", C.c_uint32), ("max_tasks", C.c_uint32),
-                ("encode_window", C.c_uint32), ("admission_bytes", C.c_uint64), ("shape", LaunchShape)]
+                ("encode_window", C.c_uint32), ("admission_bytes", C.c_uint64), ("staging_bytes", C.c_uint64),
+                ("shape", LaunchShape)]
 
 
 class ServerStats(C.Structure):

@@ -237,6 +237,44 @@ class DeviceMemory:
             pass
 
 
+class HostMemory:
+    """Pinned, mapped host memory (hipHostMalloc): device kernels read it over
+    the host link at the same address; freed on close()/garbage collection."""
+
+    def __init__(self, nbytes: int):
+        self.size = int(nbytes)
+        ptr = C.c_void_p(0)
+        if self.size:
+            _check_hip(N.lib().ts_hip_host_malloc(self.size, C.byref(ptr)), "hipHostMalloc")
+        self.ptr = ptr.value or 0
+
+    def array(self) -> np.ndarray:
+        """A uint8 NumPy view of the memory (valid while this object lives)."""
+        if not self.size:
+            return np.zeros(0, np.uint8)
+        return np.ctypeslib.as_array((C.c_uint8 * self.size).from_address(self.ptr))
+
+    def upload(self, arr, offset: int = 0) -> None:
+        a = np.ascontiguousarray(arr) if not isinstance(arr, (bytes, bytearray)) else np.frombuffer(arr, np.uint8)
+        assert offset + a.nbytes <= self.size
+        C.memmove(self.ptr + offset, a.ctypes.data, a.nbytes)
+
+    def download(self, nbytes: Optional[int] = None, offset: int = 0) -> np.ndarray:
+        n = self.size - offset if nbytes is None else nbytes
+        return self.array()[offset:offset + n].copy()
+
+    def close(self) -> None:
+        if self.ptr:
+            N.lib().ts_hip_host_free(self.ptr)
+            self.ptr = 0
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class Stream:
     def __init__(self):
         s = C.c_void_p(0)
@@ -481,7 +519,7 @@ class DecodeQueue:
 
 
 class ArchiveSet:
-    """Archives resident in device memory with their tile tables (tilestream/serve.h)."""
+    """Archives resident in device or pinned host memory with their tile tables (tilestream/serve.h)."""
 
     def __init__(self):
         handle = C.c_void_p(0)
@@ -491,9 +529,10 @@ class ArchiveSet:
         self.handle = handle.value
         self._keep = []
 
-    def add(self, device_file: DeviceMemory, metadata: Metadata, size: Optional[int] = None, offset: int = 0):
-        """Adds an archive already in device memory (at `offset` in the
-        allocation); returns (status, id)."""
+    def add(self, device_file, metadata: Metadata, size: Optional[int] = None, offset: int = 0):
+        """Adds an archive already in device memory (DeviceMemory) or in pinned
+        host memory (HostMemory; its tiles are staged per batch), at `offset`
+        in the allocation; returns (status, id)."""
         ident = C.c_uint32(0)
         n = device_file.size - offset if size is None else size
         st = Status(N.lib().ts_archive_set_add(self.handle, ConstBytes(device_file.ptr + offset, n),
@@ -526,12 +565,13 @@ class ArchiveSet:
 
 
 def server_options(streams=None, max_batch=None, max_tasks=None, admission_bytes=None, shape=None,
-                   encode_window=None) -> ServerOptions:
+                   encode_window=None, staging_bytes=None) -> ServerOptions:
     """The library's default server options with the given fields replaced."""
     o = ServerOptions()
     N.lib().ts_server_default_options(C.byref(o))
     for name, value in (("streams", streams), ("max_batch", max_batch), ("max_tasks", max_tasks),
-                        ("admission_bytes", admission_bytes), ("encode_window", encode_window)):
+                        ("admission_bytes", admission_bytes), ("encode_window", encode_window),
+                        ("staging_bytes", staging_bytes)):
         if value is not None:
             setattr(o, name, value)
     if shape is not None:

@@ -16,6 +16,34 @@ DecodeFn tsk_decode_table_mixed(std::uint32_t group, std::uint32_t block, std::u
 
 namespace {
 
+// Copies the 16-byte blocks that hold each staged task's payload from the
+// resident address (mapped host memory, read over the host link) into the
+// task's place in device staging.  Workgroups stride over the tasks.
+__global__ void __launch_bounds__(256) stage_kernel(tsk_decode_args a) {
+    using tsk::gptr;
+    using tsk::u32x4;
+    for (std::uint64_t i = blockIdx.x; i < a.task_count; i += gridDim.x) {
+        const std::uint32_t block = tsk::load_constant(a.staged, i);
+        if (block == TSK_NOT_STAGED) continue;
+        const std::uint32_t r = tsk::load_constant(a.request_index, i);
+        const tsk_request q = tsk::load_constant(a.requests, r);
+        const std::uint32_t local = static_cast<std::uint32_t>(i - q.first_task);
+        const std::uint32_t dr = local / q.cols;
+        const std::uint32_t col = q.col0 + (local - dr * q.cols);
+        const std::uint32_t row = q.row0 + dr;
+        const tsk_decode_task t =
+            tsk::load_constant(a.tasks, std::uint64_t{q.tile_base} + std::uint64_t{row} * q.tile_columns + col);
+        const std::uintptr_t first = reinterpret_cast<std::uintptr_t>(t.payload) & ~std::uintptr_t{15};
+        const std::uintptr_t end = (reinterpret_cast<std::uintptr_t>(t.payload) + std::uint64_t{t.words} * 4 + 15) &
+                                   ~std::uintptr_t{15};
+        const std::uint32_t blocks = static_cast<std::uint32_t>((end - first) / 16);
+        const gptr<const u32x4> src = tsk::global(reinterpret_cast<const u32x4 *>(first));
+        const gptr<u32x4> dst = tsk::global(reinterpret_cast<u32x4 *>(const_cast<std::uint8_t *>(a.staging)) +
+                                            std::uint64_t{block});
+        for (std::uint32_t k = threadIdx.x; k < blocks; k += blockDim.x) dst[k] = __builtin_nontemporal_load(src + k);
+    }
+}
+
 __global__ void results_begin_kernel(ResultBits *results, std::uint32_t count) {
     const std::uint64_t stride = std::uint64_t{gridDim.x} * blockDim.x;
     for (std::uint64_t i = std::uint64_t{blockIdx.x} * blockDim.x + threadIdx.x; i < count; i += stride)
@@ -83,6 +111,14 @@ int tsk_results_complete(const tsk_decode_image *images, std::uint32_t count, vo
     hipLaunchKernelGGL(results_complete_kernel, dim3(small_grid(count)), dim3(256), 0,
                        static_cast<hipStream_t>(stream), images, count, static_cast<const ResultBits *>(results),
                        static_cast<ResultBits *>(results));
+    return hipGetLastError();
+}
+
+int tsk_stage_payloads(const tsk_decode_args *a, void *stream) {
+    if (a->task_count == 0 || a->staged == nullptr) return hipSuccess;
+    const std::uint64_t grid = a->task_count < 8192 ? a->task_count : 8192;
+    hipLaunchKernelGGL(stage_kernel, dim3(static_cast<std::uint32_t>(grid)), dim3(256), 0,
+                       static_cast<hipStream_t>(stream), *a);
     return hipGetLastError();
 }
 

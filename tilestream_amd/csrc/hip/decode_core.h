@@ -498,6 +498,12 @@ __device__ inline tsk_decode_task request_task(const tsk_decode_args &a, std::ui
     t.column = col;
     t.row = row;
     request = r;
+    if (a.staged != nullptr) {
+        // A tile of an archive in host memory, copied into device staging.
+        const std::uint32_t block = load_constant(a.staged, i);
+        if (block != TSK_NOT_STAGED)
+            t.payload = a.staging + std::uint64_t{block} * 16 + (reinterpret_cast<std::uintptr_t>(t.payload) & 15u);
+    }
     return t;
 }
 

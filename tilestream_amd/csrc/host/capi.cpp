@@ -76,6 +76,8 @@ static_assert(offsetof(ts_serve_request, compressed) == offsetof(hip::ServeReque
 static_assert(offsetof(ts_serve_request, offset) == offsetof(hip::ServeRequest, offset));
 static_assert(offsetof(ts_server_options, encode_window) == offsetof(hip::ServerOptions, encode_window));
 static_assert(offsetof(ts_server_options, admission_bytes) == offsetof(hip::ServerOptions, admission_bytes));
+static_assert(offsetof(ts_server_options, staging_bytes) == offsetof(hip::ServerOptions, staging_bytes));
+static_assert(offsetof(ts_archive_info, host) == offsetof(hip::ArchiveInfo, host));
 static_assert(offsetof(ts_server_options, shape) == offsetof(hip::ServerOptions, shape));
 static_assert(offsetof(ts_archive_info, tile_count) == offsetof(hip::ArchiveInfo, tile_count));
 

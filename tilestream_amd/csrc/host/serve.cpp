@@ -19,10 +19,16 @@ namespace {
 
 struct Archive {
     ArchiveInfo info;
-    const std::uint8_t *bytes = nullptr; // device
+    const std::uint8_t *bytes = nullptr; // device-visible address (device or mapped pinned host memory)
     std::uint64_t tile_base = 0;         // first entry in the set's tile table
     std::uint32_t tile_columns = 0;
 };
+
+// 16-byte blocks holding a payload: what staging copies and the decoder reads.
+inline std::uint64_t staged_blocks(const tsk_decode_task &t) {
+    const auto p = reinterpret_cast<std::uintptr_t>(t.payload);
+    return ((p + std::uint64_t{t.words} * 4 + 15) / 16) - p / 16;
+}
 
 constexpr std::uint64_t align_up(std::uint64_t v, std::uint64_t a) { return (v + a - 1) / a * a; }
 
@@ -40,6 +46,7 @@ struct ArchiveSet::Impl {
     const std::uint16_t *device_rows = nullptr;
     std::uint64_t committed = 0;
     std::uint64_t row_count = 0;
+    bool any_host = false;
 
     ~Impl() {
         if (storage != nullptr) static_cast<void>(hipFree(storage));
@@ -80,6 +87,10 @@ Status ArchiveSet::add(ConstBytes device_file, const ImageMetadata &m, std::uint
     const std::uint64_t down = host::ceil_div(m.image.height, m.tile.height);
     if (m.tiles.size() != across * down) return Status::malformed_data;
     if (const Status st = detail::accessible(device_file.data, s.device, s.pageable); st != Status::success) return st;
+    hipPointerAttribute_t attr{};
+    bool host = false;
+    if (hipPointerGetAttributes(&attr, device_file.data) == hipSuccess) host = attr.type == hipMemoryTypeHost;
+    else static_cast<void>(hipGetLastError());
     try {
         const std::size_t tiles_before = s.tiles.size(), rows_before = s.rows.size();
         auto rollback = [&](Status st) {
@@ -115,10 +126,12 @@ Status ArchiveSet::add(ConstBytes device_file, const ImageMetadata &m, std::uint
         a.info.tile = m.tile;
         a.info.tile_count = m.tiles.size();
         a.info.file_bytes = m.file_bytes;
+        a.info.host = host ? 1u : 0u;
         a.bytes = base;
         a.tile_base = tiles_before;
         a.tile_columns = static_cast<std::uint32_t>(across);
         s.archives.push_back(a);
+        s.any_host = s.any_host || host;
         id = static_cast<std::uint32_t>(s.archives.size() - 1);
         return Status::success;
     } catch (const std::bad_alloc &) {
@@ -171,8 +184,9 @@ namespace {
 //
 // Table layout (host and device alike): work counter | DeviceResult[n]
 // (region requests first, then encode requests) | tsk_decode_image[regions] |
-// tsk_request[regions] | request of every decode task | tsk_encode_image[encodes]
-// (uint16 images, then float32) | encode window entries.
+// tsk_request[regions] | request of every decode task | staging block of every
+// decode task | tsk_encode_image[encodes] (uint16 images, then float32) |
+// encode window entries.
 struct Slot {
     hipStream_t stream = nullptr;
     hipEvent_t done = nullptr;
@@ -180,6 +194,7 @@ struct Slot {
     std::uint8_t *host_view = nullptr; // the same memory as the device addresses it
     std::uint8_t *device = nullptr;
     std::uint8_t *scratch = nullptr;   // encoder workspace (look-back buffers, carried cursor)
+    std::uint8_t *staging = nullptr;   // payloads of tiles of host archives
     std::uint64_t published = 0;       // offset of the published results in `host`
     std::vector<std::uint64_t> order;  // request index of each result of the current batch
     std::uint64_t bytes = 0;           // output bytes admitted
@@ -189,7 +204,7 @@ struct Slot {
 constexpr std::uint64_t kCounterBytes = 256;
 
 struct Layout {
-    std::uint64_t results, images, requests, index, encode, entries, total;
+    std::uint64_t results, images, requests, index, staged, encode, entries, total;
 };
 
 Layout layout_for(std::uint64_t regions, std::uint64_t encodes, std::uint64_t tasks, std::uint64_t entries) {
@@ -198,7 +213,8 @@ Layout layout_for(std::uint64_t regions, std::uint64_t encodes, std::uint64_t ta
     l.images = align_up(l.results + (regions + encodes) * sizeof(DeviceResult), 256);
     l.requests = align_up(l.images + regions * sizeof(tsk_decode_image), 256);
     l.index = align_up(l.requests + regions * sizeof(tsk_request), 256);
-    l.encode = align_up(l.index + tasks * sizeof(std::uint32_t), 256);
+    l.staged = align_up(l.index + tasks * sizeof(std::uint32_t), 256);
+    l.encode = align_up(l.staged + tasks * sizeof(std::uint32_t), 256);
     l.entries = align_up(l.encode + encodes * sizeof(tsk_encode_image), 256);
     l.total = align_up(l.entries + entries * sizeof(tsk_window_entry), 16);
     return l;
@@ -218,6 +234,7 @@ struct Extent {
     std::uint64_t tasks = 0;   // tiles decoded or encoded
     std::uint64_t bytes = 0;   // output bytes (encode: the compress bound)
     std::uint64_t decoded = 0; // region pixels x stored sample bytes, or encoded input bytes
+    std::uint64_t staged = 0;  // region of a host archive: staging bytes (16-byte blocks)
 };
 
 // Splits the tiles of consecutive images into windows of at most `window`
@@ -263,6 +280,7 @@ struct Server::Impl {
             if (s.host != nullptr) static_cast<void>(hipHostFree(s.host));
             if (s.device != nullptr) static_cast<void>(hipFree(s.device));
             if (s.scratch != nullptr) static_cast<void>(hipFree(s.scratch));
+            if (s.staging != nullptr) static_cast<void>(hipFree(s.staging));
         }
     }
 
@@ -281,7 +299,8 @@ Status Server::create(const ArchiveSet &archives, const ServerOptions &options, 
     const ArchiveSet::Impl *set = archives.impl();
     if (set == nullptr || set->committed == 0 || set->committed != set->archives.size()) return Status::invalid_argument;
     if (options.streams == 0 || options.streams > 16 || options.max_batch == 0 || options.max_tasks == 0 ||
-        options.admission_bytes == 0 || options.encode_window < 64 || options.encode_window > (1u << 20))
+        options.admission_bytes == 0 || options.encode_window < 64 || options.encode_window > (1u << 20) ||
+        options.staging_bytes < 16 || options.staging_bytes / 16 > 0xfffffffeull)
         return Status::invalid_argument;
     const LaunchShape &sh = options.shape;
     if (!tsk_decode_supported(sh.group, sh.block, sh.depth)) return Status::invalid_argument;
@@ -323,6 +342,11 @@ Status Server::create(const ArchiveSet &archives, const ServerOptions &options, 
             s.device = static_cast<std::uint8_t *>(d);
             if (hipMalloc(&w, Scratch::total(options.encode_window)) != hipSuccess) return detail::hip_failure();
             s.scratch = static_cast<std::uint8_t *>(w);
+            if (set->any_host) {
+                void *g = nullptr;
+                if (hipMalloc(&g, options.staging_bytes) != hipSuccess) return detail::hip_failure();
+                s.staging = static_cast<std::uint8_t *>(g);
+            }
             s.order.reserve(options.max_batch);
         }
         server.reset(new Server(std::move(impl)));
@@ -374,6 +398,13 @@ Status Server::Impl::extent(const ServeRequest &r, Extent &e) const noexcept {
     const std::uint64_t stored = image.type == PixelType::uint16 ? 2 : 4;
     e.decoded = area * stored;
     e.bytes = area * (r.calibrated ? 4 : stored);
+    if (a.info.host) {
+        for (std::uint32_t dr = 0; dr < e.rows; ++dr)
+            for (std::uint32_t dc = 0; dc < e.cols; ++dc)
+                e.staged += 16 * staged_blocks(set->tiles[a.tile_base + std::uint64_t{e.row0 + dr} * a.tile_columns +
+                                                          e.col0 + dc]);
+        if (e.staged > options.staging_bytes) return Status::resource_limit;
+    }
     return Status::success;
 }
 
@@ -409,6 +440,10 @@ Status Server::Impl::launch(Slot &slot, const ServeRequest *requests, std::uint6
 
     std::uint8_t *h = slot.host;
     std::memset(h, 0, l.images); // work counter and results start at zero
+    auto *staged = reinterpret_cast<std::uint32_t *>(h + l.staged);
+    bool staging = false;
+    for (std::uint64_t k = 0; k < count && !staging; ++k) staging = extents[first + k].staged != 0;
+    std::uint64_t staged_block = 0;
     auto *images = reinterpret_cast<tsk_decode_image *>(h + l.images);
     auto *reqs = reinterpret_cast<tsk_request *>(h + l.requests);
     auto *index = reinterpret_cast<std::uint32_t *>(h + l.index);
@@ -468,6 +503,18 @@ Status Server::Impl::launch(Slot &slot, const ServeRequest *requests, std::uint6
         q.cols = e.cols;
         q.reserved = 0;
         std::fill_n(index + task, e.tasks, static_cast<std::uint32_t>(j));
+        if (staging) {
+            // Tiles of a host archive get consecutive 16-byte blocks of staging.
+            for (std::uint64_t k = 0; k < e.tasks; ++k) {
+                if (!a.info.host) {
+                    staged[task + k] = TSK_NOT_STAGED;
+                    continue;
+                }
+                const std::uint64_t dr = k / e.cols, dc = k % e.cols;
+                staged[task + k] = static_cast<std::uint32_t>(staged_block);
+                staged_block += staged_blocks(set->tiles[a.tile_base + (e.row0 + dr) * a.tile_columns + e.col0 + dc]);
+            }
+        }
         task += e.tasks;
         slot.order[j] = first + k;
         ++j;
@@ -508,6 +555,11 @@ Status Server::Impl::launch(Slot &slot, const ServeRequest *requests, std::uint6
         args.request_count = static_cast<std::uint32_t>(regions);
         args.requests = reinterpret_cast<const tsk_request *>(slot.device + l.requests);
         args.request_index = reinterpret_cast<const std::uint32_t *>(slot.device + l.index);
+        if (staging) {
+            args.staged = reinterpret_cast<const std::uint32_t *>(slot.device + l.staged);
+            args.staging = slot.staging;
+            if (tsk_stage_payloads(&args, slot.stream) != hipSuccess) return detail::hip_failure();
+        }
         if (tsk_decode(&args, slot.stream) != hipSuccess ||
             tsk_results_publish(args.images, static_cast<std::uint32_t>(regions), args.results,
                                 slot.host_view + slot.published, slot.stream) != hipSuccess)
@@ -639,13 +691,15 @@ Status Server::replay(const ServeRequest *requests, const std::uint64_t *arrival
                 if (slot.busy || dispatched == arrived) continue;
                 // Admit arrived requests in order while the batch and the
                 // device have room; one request is always admitted alone.
-                std::uint64_t n = 0, tasks = 0, bytes = 0;
+                std::uint64_t n = 0, tasks = 0, bytes = 0, staged = 0;
                 while (dispatched + n < arrived && n < s.options.max_batch) {
                     const Extent &e = s.extents[dispatched + n];
                     if (tasks + e.tasks > s.options.max_tasks) break;
+                    if (staged + e.staged > s.options.staging_bytes) break;
                     if (in_flight + bytes + e.bytes > s.options.admission_bytes && (n != 0 || in_flight != 0)) break;
                     tasks += e.tasks;
                     bytes += e.bytes;
+                    staged += e.staged;
                     ++n;
                 }
                 if (n == 0) break;
