@@ -482,6 +482,202 @@ def serve_mix_leg(args, threads: int, stream):
     }
 
 
+def staged_leg(args, threads: int, stream):
+    """Region requests over archives in pinned host memory: 4096x4096 uint16
+    archives (64x64 tiles), sides log-uniform in [32, 2048], fused (x - 100) *
+    1.05 to FP32.  The server stages the tiles each batch touches into device
+    memory with a kernel reading over the host link while other batches
+    decode.  Beside it, the reference's protocol on the same requests (host
+    gather of the touched tiles into pinned staging, one H2D copy per request
+    on a copy stream, prepare_batch with tile mappings, decode on a compute
+    stream that waits for the copy) and a plain 1 GiB H2D copy."""
+    from tilestream_amd import api, generators
+    from tilestream_amd.workloads import ResidentBatch, _align
+
+    side, tile = 4096, (64, 64)
+    offset, gain = 100.0, 1.05
+    t0 = time.perf_counter()
+    rb = ResidentBatch.build([(side, side, np.uint16, tile)] * args.staged_archives, args.seed + 30, threads)
+    host_off, total = [], 0
+    for e in rb.entries:
+        host_off.append(total)
+        total += _align(len(e.archive))
+    host = api.HostMemory(total)
+    for e, off in zip(rb.entries, host_off):
+        host.upload(np.frombuffer(e.archive, np.uint8), off)
+    aset = api.ArchiveSet()
+    for e, off in zip(rb.entries, host_off):
+        st, _ = aset.add(host, e.meta, len(e.archive), off)
+        assert st == api.Status.success, st
+    assert aset.commit(stream) == api.Status.success
+    n = args.staged_requests
+    rois = generators.roi_requests(n, args.seed + 31, side, side, archives=args.staged_archives)
+    align = args.pitch_align
+    offsets, pitches, out_total = [], [], 0
+    for a, x, y, w, h in rois:
+        pitch = (w * 4 + align - 1) // align * align
+        offsets.append(out_total)
+        pitches.append(pitch)
+        out_total += (pitch * (h - 1) + w * 4 + 255) // 256 * 256
+    out = api.DeviceMemory(out_total)
+    requests = []
+    for (a, x, y, w, h), off, pitch in zip(rois, offsets, pitches):
+        view = api.device_view(out, api.PixelType.float32, w, h, pitch, offset=off, capacity=pitch * (h - 1) + w * 4)
+        requests.append(api.region_request(a, (x, y, w, h), view, True, offset, gain))
+    # Tiles each request touches, their payload bytes, and the 16-byte blocks staged.
+    cols = rows_ = side // tile[0]
+    tile_off = [np.array([e.meta.tile(k).offset for k in range(cols * rows_)], np.int64).reshape(rows_, cols)
+                for e in rb.entries]
+    tile_len = [np.array([e.meta.tile(k).length for k in range(cols * rows_)], np.int64).reshape(rows_, cols)
+                for e in rb.entries]
+    touched, blocks = [], 0
+    for a, x, y, w, h in rois:
+        r0, r1, c0, c1 = y // tile[1], (y + h - 1) // tile[1] + 1, x // tile[0], (x + w - 1) // tile[0] + 1
+        touched.append((r0, r1, c0, c1))
+        o = tile_off[a][r0:r1, c0:c1] + host_off[a]
+        ln = tile_len[a][r0:r1, c0:c1]
+        blocks += int(((o + ln + 15) // 16 - o // 16).sum())
+    payload = sum(int(tile_len[a][r0:r1, c0:c1].sum()) for (a, *_), (r0, r1, c0, c1) in zip(rois, touched))
+    decoded = sum(w * h * 2 for *_, w, h in rois)
+    log(f"staged: {len(rb.entries)} archives in pinned host memory ({rb.archive_bytes / 1e9:.2f} GB), {n} requests, "
+        f"{decoded / 1e9:.2f} GB decoded, {payload / 1e9:.3f} GB of touched payload, set up in "
+        f"{time.perf_counter() - t0:.1f} s")
+    del rb.src  # the device copy of the archives is not used here
+
+    shape = tuple(int(v) for v in args.roi_shape.split(","))
+    server = api.Server(aset, api.server_options(streams=args.roi_streams, max_batch=args.roi_max_batch,
+                                                 staging_bytes=args.staging_mib << 20, shape=shape))
+    sat = None
+    for _ in range(2):
+        st, lat, statuses, _, stats = server.replay(requests, np.zeros(n, np.uint64))
+        assert st == api.Status.success and (statuses == 0).all(), st
+        sat = stats
+    log(f"staged saturation: {sat.seconds * 1e3:.2f} ms, {n / sat.seconds:.0f} requests/s, "
+        f"{sat.decoded_bytes / sat.seconds / 1e9:.1f} GB/s decoded, {blocks * 16 / sat.seconds / 1e9:.1f} GB/s "
+        f"over the host link, {sat.batches} batches")
+    rng = np.random.default_rng(args.seed + 32)
+    latency = []
+    for q in args.staged_qps:
+        arrivals = np.cumsum(rng.exponential(1e9 / q, n)).astype(np.uint64)
+        st, lat, statuses, _, stats = server.replay(requests, arrivals)
+        assert st == api.Status.success and (statuses == 0).all(), st
+        rec = {"qps": q, "achieved_qps": round(n / (arrivals[-1] * 1e-9), 1),
+               "p50_us": round(percentile(lat, 50) / 1e3, 1), "p99_us": round(percentile(lat, 99) / 1e3, 1),
+               "batches": int(stats.batches)}
+        log(f"staged latency at {q} req/s: p50 {rec['p50_us']} us, p99 {rec['p99_us']} us")
+        latency.append(rec)
+
+    # Plain H2D copy of 1 GiB of pinned memory: the host link's ceiling here.
+    big = api.HostMemory(1 << 30)
+    dev = api.DeviceMemory(1 << 30)
+    copy_ms = timed(stream, lambda: api.memcpy_async(dev.ptr, big.ptr, 1 << 30, stream), 3)
+    del big, dev
+    link = (1 << 30) / (copy_ms * 1e-3) / 1e9
+
+    # The reference's protocol on a sample (benchmarks/codec_benchmark.cpp archive_roi).
+    sample = min(args.staged_plain_sample, n)
+    sizes = [int(tile_len[a][r0:r1, c0:c1].sum()) for (a, *_), (r0, r1, c0, c1) in zip(rois[:sample], touched[:sample])]
+    st_off = np.concatenate([[0], np.cumsum([_align(z) for z in sizes])]).astype(np.int64)
+    staging_host = api.HostMemory(int(st_off[-1]))
+    staging_dev = api.DeviceMemory(int(st_off[-1]))
+    hview, sview = host.array(), staging_host.array()
+    copy_stream = api.Stream()
+    res = api.DeviceMemory(16 * sample)
+    ws = None
+    t_start = time.perf_counter()
+    items, keep = [], []
+    for i in range(sample):
+        a, x, y, w, h = rois[i]
+        r0, r1, c0, c1 = touched[i]
+        at, maps = int(st_off[i]), []
+        for r in range(r0, r1):
+            for c in range(c0, c1):
+                o, ln = int(tile_off[a][r, c]) + host_off[a], int(tile_len[a][r, c])
+                sview[at:at + ln] = hview[o:o + ln]
+                maps.append(api.TileMapping(r * cols + c, at - int(st_off[i])))
+                at += ln
+        it = api.BatchItem()
+        it.roi = api.Roi(x, y, w, h)
+        it.output = requests[i].output
+        it.calibration.offset = api.scalar(offset)
+        it.calibration.gain = api.scalar(gain)
+        arr = (api.TileMapping * len(maps))(*maps)
+        it.compressed_input.bytes = api.ConstBytes(staging_dev.ptr + int(st_off[i]), sizes[i])
+        it.compressed_input.metadata = rb.entries[a].meta.handle
+        it.compressed_input.mappings = C_cast_mappings(arr)
+        it.compressed_input.mapping_count = len(maps)
+        items.append(it)
+        keep.append(arr)
+    t_gather = time.perf_counter()
+    plans = []
+    for it in items:
+        st, plan = api.prepare_batch(api.Operation.decode_calibrate, [it], stream=stream)
+        assert st == api.Status.success, st
+        plans.append(plan)
+        if ws is None or plan.workspace_bytes() > ws.size:
+            ws = api.DeviceMemory(max(256, plan.workspace_bytes()))
+    t_prepare = time.perf_counter()
+    copied = [api.Event() for _ in range(sample)]
+    first, last = api.Event(), api.Event()
+    first.record(copy_stream)
+    for i in range(sample):
+        api.memcpy_async(staging_dev.ptr + int(st_off[i]), staging_host.ptr + int(st_off[i]), sizes[i], copy_stream)
+        copied[i].record(copy_stream)
+        stream.wait(copied[i])
+        assert api.decode_calibrate_async(plans[i], ws, res.ptr + 16 * i, 1, stream) == api.Status.success
+    last.record(stream)
+    last.synchronize()
+    t_end = time.perf_counter()
+    plain_decoded = sum(w * h * 2 for *_, w, h in rois[:sample])
+    plain = {"requests": sample, "host_end_to_end_ms": round((t_end - t_start) * 1e3, 2),
+             "gather_ms": round((t_gather - t_start) * 1e3, 2), "prepare_ms": round((t_prepare - t_gather) * 1e3, 2),
+             "device_ms": round(first.elapsed_ms(last), 3),
+             "requests_per_s": round(sample / (t_end - t_start), 1),
+             "decoded_GBps_end_to_end": round(plain_decoded / (t_end - t_start) / 1e9, 2),
+             "decoded_GBps_device": round(plain_decoded / (first.elapsed_ms(last) * 1e-3) / 1e9, 2),
+             "note": "host gather (NumPy slices) + prepare_batch with tile mappings + per-request H2D on a copy "
+                     "stream overlapped with decode on the compute stream"}
+    status, _ = api.read_results(res, sample)
+    log(f"staged reference protocol ({sample} requests): {plain['host_end_to_end_ms']} ms end to end "
+        f"({plain['requests_per_s']} req/s), device span {plain['device_ms']} ms; host link {link:.1f} GB/s")
+
+    verified = None
+    if not args.no_verify:
+        out.fill(0)
+        st, _, statuses, _, _ = server.replay(requests, np.zeros(n, np.uint64))
+        ok = st == api.Status.success and bool((statuses == 0).all()) and bool((status == 0).all())
+        o32, g32 = np.float32(offset), np.float32(gain)
+        for i in sorted(set(range(0, n, max(1, n // 100))) | {n - 1}):
+            a, x, y, w, h = rois[i]
+            want = ((rb.entries[a].pixels[y:y + h, x:x + w].astype(np.float32) - o32) * g32).view(np.uint32)
+            pitch = pitches[i]
+            raw = np.zeros(pitch * h, np.uint8)
+            raw[:pitch * (h - 1) + w * 4] = out.download(pitch * (h - 1) + w * 4, offsets[i])
+            got = np.ascontiguousarray(raw.reshape(h, pitch)[:, :w * 4]).view(np.uint32)
+            ok = ok and np.array_equal(got, want)
+        verified = bool(ok)
+        log(f"staged verification bit-exact: {verified}")
+    return {
+        "workload": f"{len(rb.entries)} 4096x4096 uint16 archives (64x64 tiles) in pinned host memory, {n} region "
+                    f"requests, sides log-uniform in [32, 2048], fused (x - {offset:g}) * {gain:g} to FP32",
+        "requests": n, "decoded_bytes": decoded, "touched_payload_bytes": payload, "staged_bytes": blocks * 16,
+        "saturation": {"ms": round(sat.seconds * 1e3, 3), "requests_per_s": round(n / sat.seconds, 1),
+                       "decoded_GBps": round(sat.decoded_bytes / sat.seconds / 1e9, 2),
+                       "host_link_GBps": round(blocks * 16 / sat.seconds / 1e9, 2), "batches": int(sat.batches)},
+        "host_link_copy_GBps": round(link, 2),
+        "latency": latency,
+        "reference_protocol": plain,
+        "server": {"streams": args.roi_streams, "max_batch": args.roi_max_batch, "staging_mib": args.staging_mib},
+        "verified_bit_exact": verified,
+    }
+
+
+def C_cast_mappings(arr):
+    import ctypes
+    from tilestream_amd import api
+    return ctypes.cast(arr, ctypes.POINTER(api.TileMapping))
+
+
 def fp64_errors(got_bits: np.ndarray, x: np.ndarray, offset: np.ndarray, gain: np.ndarray):
     """ULP error of FP32 (x - offset) * gain against the FP64 value, over
     finite normal references: max / p99 / median, and results beyond the
@@ -597,6 +793,14 @@ def main() -> int:
     p.add_argument("--qps", type=lambda v: [float(x) for x in v.split(",")], default=[100000.0, 400000.0],
                    help="Poisson arrival rates (requests/s) for the ROI latency runs")
     p.add_argument("--no-mix", action="store_true", help="skip the serving-mix leg")
+    p.add_argument("--no-staged", action="store_true", help="skip the host-archive (staged) leg")
+    p.add_argument("--staged-only", action="store_true", help="only the host-archive (staged) leg")
+    p.add_argument("--staged-archives", type=int, default=16)
+    p.add_argument("--staged-requests", type=int, default=3000)
+    p.add_argument("--staged-plain-sample", type=int, default=300)
+    p.add_argument("--staging-mib", type=int, default=256, help="server staging per stream")
+    p.add_argument("--staged-qps", type=lambda v: [float(x) for x in v.split(",")], default=[20000.0, 80000.0],
+                   help="Poisson arrival rates (requests/s) for the staged latency runs")
     p.add_argument("--no-encode", action="store_true", help="skip the encode leg")
     p.add_argument("--encode-only", action="store_true", help="only the encode leg (of the decode batch's images)")
     p.add_argument("--mix-only", action="store_true", help="only the serving-mix leg")
@@ -627,6 +831,10 @@ def main() -> int:
         log(f"copy probe: {args.steps} copies of {nbytes} bytes (read {nbytes}, write {nbytes} each)")
         return 0
     threads = max(1, min(16, os.cpu_count() or 1))
+    if args.staged_only:
+        staged = staged_leg(args, threads, api.Stream())
+        print(json.dumps({"metric": "staged region serving from pinned host memory", "staged": staged}), flush=True)
+        return 0 if staged["verified_bit_exact"] in (None, True) else 3
     if args.mix_only:
         mix = serve_mix_leg(args, threads, api.Stream())
         print(json.dumps({"metric": "serving mix throughput and latency", "serve_mix": mix}), flush=True)
@@ -742,6 +950,7 @@ def main() -> int:
     fused = None if args.no_fused else fused_leg(args, stream)
     enc = None if args.no_encode else encode_leg(args, batch, stream)
     mix = None if args.no_mix else serve_mix_leg(args, threads, stream)
+    staged = None if args.no_staged else staged_leg(args, threads, stream)
     if cpu:
         log(f"cpu baseline {cpu['value']} GB/s on {cpu['cores']} threads ({cpu['kind']})")
 
@@ -793,12 +1002,13 @@ def main() -> int:
         "fused_real_shape": fused,
         "encode": enc,
         "serve_mix": mix,
+        "staged": staged,
     }
     if roi is not None:
         out["p99_roi_latency_us"] = {str(int(r["qps"])): r["p99_us"] for r in roi["latency"]}
     print(json.dumps(out), flush=True)
     ok = verified in (None, True) and all(leg is None or leg["verified_bit_exact"] in (None, True)
-                                          for leg in (roi, mix, enc))
+                                          for leg in (roi, mix, enc, staged))
     return 0 if ok else 3
 
 

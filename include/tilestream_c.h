@@ -217,6 +217,7 @@ int ts_hip_event_create(void **event);
 int ts_hip_event_destroy(void *event);
 int ts_hip_event_record(void *event, void *stream);
 int ts_hip_event_synchronize(void *event);
+int ts_hip_stream_wait_event(void *stream, void *event);
 int ts_hip_event_elapsed_ms(void *start, void *stop, float *ms);
 /* Sizes of the record types, for binding checks: name -> sizeof, 0 if unknown. */
 uint64_t ts_abi_sizeof(const char *name);

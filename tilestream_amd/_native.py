@@ -236,6 +236,7 @@ PROTOTYPES = {
     "ts_hip_event_destroy": (_int, [_vp]),
     "ts_hip_event_record": (_int, [_vp, _vp]),
     "ts_hip_event_synchronize": (_int, [_vp]),
+    "ts_hip_stream_wait_event": (_int, [_vp, _vp]),
     "ts_hip_event_elapsed_ms": (_int, [_vp, _vp, _P(C.c_float)]),
     "ts_abi_sizeof": (_u64, [C.c_char_p]),
     "tsk_copy_blocks": (_int, [_vp, _vp, _u64, _vp]),

@@ -284,6 +284,10 @@ class Stream:
     def synchronize(self) -> None:
         _check_hip(N.lib().ts_hip_stream_synchronize(self.handle), "hipStreamSynchronize")
 
+    def wait(self, event: "Event") -> None:
+        """Later work on this stream waits for `event`."""
+        _check_hip(N.lib().ts_hip_stream_wait_event(self.handle, event.handle), "hipStreamWaitEvent")
+
     def close(self) -> None:
         if self.handle:
             N.lib().ts_hip_stream_destroy(self.handle)
@@ -319,6 +323,11 @@ class Event:
                 N.lib().ts_hip_event_destroy(self.handle)
         except Exception:
             pass
+
+
+def memcpy_async(dst: int, src: int, nbytes: int, stream=None) -> None:
+    """hipMemcpyAsync between any two addresses (pinned host memory makes H2D asynchronous)."""
+    _check_hip(N.lib().ts_hip_memcpy_async(dst, src, nbytes, _stream(stream)), "hipMemcpyAsync")
 
 
 def _stream(stream):

@@ -409,6 +409,9 @@ int ts_hip_event_record(void *event, void *stream) {
     return hipEventRecord(static_cast<hipEvent_t>(event), stream_of(stream));
 }
 int ts_hip_event_synchronize(void *event) { return hipEventSynchronize(static_cast<hipEvent_t>(event)); }
+int ts_hip_stream_wait_event(void *stream, void *event) {
+    return hipStreamWaitEvent(stream_of(stream), static_cast<hipEvent_t>(event), 0);
+}
 int ts_hip_event_elapsed_ms(void *start, void *stop, float *ms) {
     return hipEventElapsedTime(ms, static_cast<hipEvent_t>(start), static_cast<hipEvent_t>(stop));
 }
