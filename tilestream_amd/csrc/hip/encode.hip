@@ -115,7 +115,9 @@ __device__ inline std::uint64_t wave_sum(std::uint64_t v) {
     return v;
 }
 
-// Copies the tile into LDS, row r at px + r * stride, one 32-bit word per sample.
+// Copies the tile into LDS, row r at px + r * stride, one 32-bit word per
+// sample.  Every load of a thread is issued before the first one is waited
+// for (a tile is at most 4 chunks of 16 bytes, or 16 samples, per thread).
 template <bool U16>
 __device__ inline void stage_tile(const tsk_encode_image &img, const Geometry &g, std::uint32_t stride, std::uint32_t *px) {
     constexpr std::uint32_t s = U16 ? 2u : 4u;
@@ -124,35 +126,60 @@ __device__ inline void stage_tile(const tsk_encode_image &img, const Geometry &g
     if (vector) {
         // Rows start 16-byte aligned (tile widths are multiples of 64 samples).
         constexpr std::uint32_t per = 16u / s;
+        constexpr std::uint32_t most = 4096u / per / kThreads; // chunks per thread
         const std::uint32_t across = (g.w + per - 1u) / per;
         const std::uint32_t chunks = across * g.h;
         const std::uint32_t magic = magic_of(across);
-        for (std::uint32_t k = threadIdx.x; k < chunks; k += kThreads) {
+        u32x4 v[most];
+#pragma unroll
+        for (std::uint32_t q = 0; q < most; ++q) {
+            const std::uint32_t k = threadIdx.x + q * kThreads;
             const std::uint32_t r = div_magic(k, magic, across), c = k - r * across;
-            const std::uint8_t *p = origin + std::uint64_t{r} * img.in_pitch + c * 16u;
+            // Samples are read once: non-temporal loads.
+            if (k < chunks && (c + 1u) * per <= g.w)
+                v[q] = __builtin_nontemporal_load(reinterpret_cast<gptr<const u32x4>>(
+                    global(origin + std::uint64_t{r} * img.in_pitch + c * 16u)));
+        }
+#pragma unroll
+        for (std::uint32_t q = 0; q < most; ++q) {
+            const std::uint32_t k = threadIdx.x + q * kThreads;
+            if (k >= chunks) continue;
+            const std::uint32_t r = div_magic(k, magic, across), c = k - r * across;
             std::uint32_t *dst = px + r * stride + c * per;
             if ((c + 1u) * per <= g.w) {
-                // Samples are read once: non-temporal loads.
-                const u32x4 v = __builtin_nontemporal_load(reinterpret_cast<gptr<const u32x4>>(global(p)));
                 if constexpr (U16) {
-                    reinterpret_cast<u32x4 *>(dst)[0] = u32x4{v.x & 0xffffu, v.x >> 16, v.y & 0xffffu, v.y >> 16};
-                    reinterpret_cast<u32x4 *>(dst)[1] = u32x4{v.z & 0xffffu, v.z >> 16, v.w & 0xffffu, v.w >> 16};
+                    reinterpret_cast<u32x4 *>(dst)[0] = u32x4{v[q].x & 0xffffu, v[q].x >> 16, v[q].y & 0xffffu, v[q].y >> 16};
+                    reinterpret_cast<u32x4 *>(dst)[1] = u32x4{v[q].z & 0xffffu, v[q].z >> 16, v[q].w & 0xffffu, v[q].w >> 16};
                 } else {
-                    reinterpret_cast<u32x4 *>(dst)[0] = v;
+                    reinterpret_cast<u32x4 *>(dst)[0] = v[q];
                 }
             } else {
                 // The image's right edge: only the samples inside it.
-                for (std::uint32_t q = 0; c * per + q < g.w; ++q)
-                    dst[q] = U16 ? reinterpret_cast<gptr<const std::uint16_t>>(global(p))[q] : reinterpret_cast<gptr<const std::uint32_t>>(global(p))[q];
+                const std::uint8_t *p = origin + std::uint64_t{r} * img.in_pitch + c * 16u;
+                for (std::uint32_t e = 0; c * per + e < g.w; ++e)
+                    dst[e] = U16 ? reinterpret_cast<gptr<const std::uint16_t>>(global(p))[e]
+                                 : reinterpret_cast<gptr<const std::uint32_t>>(global(p))[e];
             }
         }
     } else {
+        constexpr std::uint32_t most = 4096u / kThreads; // samples per thread
         const std::uint32_t n = g.w * g.h;
         const std::uint32_t magic = magic_of(g.w);
-        for (std::uint32_t k = threadIdx.x; k < n; k += kThreads) {
+        std::uint32_t v[most];
+#pragma unroll
+        for (std::uint32_t q = 0; q < most; ++q) {
+            const std::uint32_t k = threadIdx.x + q * kThreads;
             const std::uint32_t r = div_magic(k, magic, g.w), c = k - r * g.w;
             const std::uint8_t *p = origin + std::uint64_t{r} * img.in_pitch;
-            px[r * stride + c] = U16 ? reinterpret_cast<gptr<const std::uint16_t>>(global(p))[c] : reinterpret_cast<gptr<const std::uint32_t>>(global(p))[c];
+            if (k < n)
+                v[q] = U16 ? reinterpret_cast<gptr<const std::uint16_t>>(global(p))[c]
+                           : reinterpret_cast<gptr<const std::uint32_t>>(global(p))[c];
+        }
+#pragma unroll
+        for (std::uint32_t q = 0; q < most; ++q) {
+            const std::uint32_t k = threadIdx.x + q * kThreads;
+            const std::uint32_t r = div_magic(k, magic, g.w), c = k - r * g.w;
+            if (k < n) px[r * stride + c] = v[q];
         }
     }
 }
@@ -235,6 +262,7 @@ template <bool U16> __global__ void __launch_bounds__(kThreads) encode_kernel(ts
     const std::uint64_t carried = (e.flags & 1u) ? a.carry[a.parity] : payload_start(img);
     if (threadIdx.x == 0)
         store_state(&state[t], segment_start ? (kFlagPrefix | (carried & kFailed) | length) : (kFlagAggregate | length));
+
     if (mode == 2) {
         // The payload is built in LDS before the offset is known.  Words of each
         // row: seed, width, packed residuals; row_word is their exclusive prefix.
