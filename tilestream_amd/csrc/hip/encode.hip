@@ -184,7 +184,8 @@ __device__ inline void stage_tile(const tsk_encode_image &img, const Geometry &g
     }
 }
 
-template <bool U16> __global__ void __launch_bounds__(kThreads) encode_kernel(tsk_encode_args a) {
+template <bool U16>
+__global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(8))) encode_kernel(tsk_encode_args a) {
     __shared__ __attribute__((aligned(16))) std::uint32_t px[kMaxTileWords];
     __shared__ std::uint32_t row_word[65];
     __shared__ std::uint32_t row_bits[64];
