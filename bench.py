@@ -672,6 +672,113 @@ def staged_leg(args, threads: int, stream):
     }
 
 
+def file_leg(args, stream):
+    """File in the page cache to calibrated FP32 on the device, per the
+    reference's file tier (raw file: read, H2D, calibrate; .tsc file: read,
+    parse and plan, H2D, decode fused with (x - 100) * 1.25): a 4096x4096
+    uint16 detector frame and a 4096x4096 float32 field, 128x32 tiles.  Each
+    stage is timed on the host clock (device stages end with a stream
+    synchronize), median of the repetitions; the two results are compared
+    word for word."""
+    import tempfile
+    from tilestream_amd import api, generators
+    side, tile = 4096, (128, 32)
+    offset, gain = 100.0, 1.25
+    reps = max(3, args.file_reps)
+    out = {}
+    with tempfile.TemporaryDirectory(dir="/tmp") as d:
+        for kind, dtype in (("u16", np.uint16), ("f32", np.uint32)):
+            px = generators.image_for(side, side, dtype, args.seed + 40)
+            st, archive = api.cpu_compress(px, api.TileShape(*tile))
+            assert st == api.Status.success, st
+            raw_path, tsc_path = os.path.join(d, f"{kind}.raw"), os.path.join(d, f"{kind}.tsc")
+            px.tofile(raw_path)
+            with open(tsc_path, "wb") as f:
+                f.write(archive)
+            ptype = api.pixel_type_of(px.dtype)
+            host = api.HostMemory(max(px.nbytes, len(archive)))
+            hv = host.array()
+            pixels = api.DeviceMemory(px.nbytes)
+            src = api.DeviceMemory(len(archive))
+            out_raw, out_tsc = api.DeviceMemory(side * side * 4), api.DeviceMemory(side * side * 4)
+            res = api.DeviceMemory(16)
+
+            def read(path, n):
+                with open(path, "rb", buffering=0) as f:
+                    got = f.readinto(memoryview(hv)[:n])
+                assert got == n
+
+            read(raw_path, px.nbytes)  # warm page cache
+            read(tsc_path, len(archive))
+            # Raw path: the calibration plan is made once, outside the timed stages.
+            it = api.BatchItem()
+            it.input = api.device_view(pixels, ptype, side, side, const=True)
+            it.output = api.device_view(out_raw, api.PixelType.float32, side, side)
+            it.calibration.offset = api.scalar(offset)
+            it.calibration.gain = api.scalar(gain)
+            st, cal_plan = api.prepare_batch(api.Operation.calibrate, [it], stream=stream)
+            assert st == api.Status.success, st
+            ws = api.DeviceMemory(max(256, cal_plan.workspace_bytes()))
+            raw_t = {"read": [], "h2d": [], "calibrate": []}
+            for _ in range(reps):
+                t0 = time.perf_counter()
+                read(raw_path, px.nbytes)
+                t1 = time.perf_counter()
+                api.memcpy_async(pixels.ptr, host.ptr, px.nbytes, stream)
+                stream.synchronize()
+                t2 = time.perf_counter()
+                assert api.calibrate_async(cal_plan, ws, res, 1, stream) == api.Status.success
+                stream.synchronize()
+                t3 = time.perf_counter()
+                raw_t["read"].append(t1 - t0)
+                raw_t["h2d"].append(t2 - t1)
+                raw_t["calibrate"].append(t3 - t2)
+            tsc_t = {"read": [], "parse_prepare": [], "h2d": [], "decode_calibrate": []}
+            ws2 = None
+            for _ in range(reps):
+                t0 = time.perf_counter()
+                read(tsc_path, len(archive))
+                t1 = time.perf_counter()
+                st, meta = api.parse_metadata(hv[:len(archive)])
+                assert st == api.Status.success, st
+                it2 = api.BatchItem()
+                it2.compressed_input.bytes = api.ConstBytes(src.ptr, len(archive))
+                it2.compressed_input.metadata = meta.handle
+                it2.roi = api.Roi(0, 0, side, side)
+                it2.output = api.device_view(out_tsc, api.PixelType.float32, side, side)
+                it2.calibration.offset = api.scalar(offset)
+                it2.calibration.gain = api.scalar(gain)
+                st, plan = api.prepare_batch(api.Operation.decode_calibrate, [it2], stream=stream)
+                assert st == api.Status.success, st
+                if ws2 is None:
+                    ws2 = api.DeviceMemory(max(256, plan.workspace_bytes()))
+                t2 = time.perf_counter()
+                api.memcpy_async(src.ptr, host.ptr, len(archive), stream)
+                stream.synchronize()
+                t3 = time.perf_counter()
+                assert api.decode_calibrate_async(plan, ws2, res, 1, stream) == api.Status.success
+                stream.synchronize()
+                t4 = time.perf_counter()
+                tsc_t["read"].append(t1 - t0)
+                tsc_t["parse_prepare"].append(t2 - t1)
+                tsc_t["h2d"].append(t3 - t2)
+                tsc_t["decode_calibrate"].append(t4 - t3)
+            same = bool(np.array_equal(out_raw.download(), out_tsc.download()))
+            med = lambda v: round(float(np.median(v)) * 1e3, 3)  # noqa: E731
+            rec = {"ratio": round(px.nbytes / len(archive), 3),
+                   "raw_file": {k: med(v) for k, v in raw_t.items()},
+                   "tsc_file": {k: med(v) for k, v in tsc_t.items()},
+                   "identical_outputs": same}
+            rec["raw_file"]["host_end_to_end"] = round(sum(rec["raw_file"].values()), 3)
+            rec["tsc_file"]["host_end_to_end"] = round(sum(rec["tsc_file"].values()), 3)
+            log(f"file tier {kind}: raw {rec['raw_file']['host_end_to_end']} ms, tsc {rec['tsc_file']['host_end_to_end']} "
+                f"ms ({rec['tsc_file']}), identical {same}")
+            out[kind] = rec
+    return {"workload": "4096x4096 uint16 frame and float32 field, 128x32 tiles, file in the page cache to FP32 "
+                        "(x - 100) * 1.25 on the device; stage medians in ms", **out,
+            "verified_bit_exact": all(r["identical_outputs"] for r in out.values())}
+
+
 def C_cast_mappings(arr):
     import ctypes
     from tilestream_amd import api
@@ -794,6 +901,9 @@ def main() -> int:
                    help="Poisson arrival rates (requests/s) for the ROI latency runs")
     p.add_argument("--no-mix", action="store_true", help="skip the serving-mix leg")
     p.add_argument("--no-staged", action="store_true", help="skip the host-archive (staged) leg")
+    p.add_argument("--no-file", action="store_true", help="skip the file-to-result leg")
+    p.add_argument("--file-only", action="store_true", help="only the file-to-result leg")
+    p.add_argument("--file-reps", type=int, default=7)
     p.add_argument("--staged-only", action="store_true", help="only the host-archive (staged) leg")
     p.add_argument("--staged-archives", type=int, default=16)
     p.add_argument("--staged-requests", type=int, default=3000)
@@ -831,6 +941,10 @@ def main() -> int:
         log(f"copy probe: {args.steps} copies of {nbytes} bytes (read {nbytes}, write {nbytes} each)")
         return 0
     threads = max(1, min(16, os.cpu_count() or 1))
+    if args.file_only:
+        fl = file_leg(args, api.Stream())
+        print(json.dumps({"metric": "file in page cache to calibrated FP32", "file": fl}), flush=True)
+        return 0 if fl["verified_bit_exact"] else 3
     if args.staged_only:
         staged = staged_leg(args, threads, api.Stream())
         print(json.dumps({"metric": "staged region serving from pinned host memory", "staged": staged}), flush=True)
@@ -951,6 +1065,7 @@ def main() -> int:
     enc = None if args.no_encode else encode_leg(args, batch, stream)
     mix = None if args.no_mix else serve_mix_leg(args, threads, stream)
     staged = None if args.no_staged else staged_leg(args, threads, stream)
+    fl = None if args.no_file else file_leg(args, stream)
     if cpu:
         log(f"cpu baseline {cpu['value']} GB/s on {cpu['cores']} threads ({cpu['kind']})")
 
@@ -1003,12 +1118,13 @@ def main() -> int:
         "encode": enc,
         "serve_mix": mix,
         "staged": staged,
+        "file": fl,
     }
     if roi is not None:
         out["p99_roi_latency_us"] = {str(int(r["qps"])): r["p99_us"] for r in roi["latency"]}
     print(json.dumps(out), flush=True)
     ok = verified in (None, True) and all(leg is None or leg["verified_bit_exact"] in (None, True)
-                                          for leg in (roi, mix, enc, staged))
+                                          for leg in (roi, mix, enc, staged, fl))
     return 0 if ok else 3
 
 

@@ -102,6 +102,43 @@ def test_parser_tables_match_oracle(orc):
                 sg.seed, sg.bit_width, sg.payload_offset, sg.payload_bytes)
 
 
+def test_threaded_parser_matches_oracle(orc):
+    """Archives of several MiB are validated and tabulated by several host
+    threads over contiguous tile ranges: same tables and statuses as the
+    oracle's sequential walk, for mutations in every range and the index."""
+    rng = np.random.default_rng(71)
+    archives = [orc.compress(g.field_f32(2048, 1536, 72), (64, 64)),
+                orc.compress(g.detector_u16(4096, 2048, 73), (256, 16))]
+    for data in archives:
+        assert len(data) > (8 << 20)
+        st, meta = ts.parse_metadata(data)
+        ost, info, tiles, segs = orc.parse(data)
+        assert st == ost == 0
+        assert (meta.summary.tile_count, meta.summary.segment_count) == (info.tile_count, info.segment_count)
+        for i in list(range(0, len(tiles), 7)) + [len(tiles) - 1]:
+            m, t = meta.tile(i), tiles[i]
+            assert (m.mode, m.width, m.height, m.offset, m.length, m.first_segment) == (
+                t.mode, t.width, t.height, t.offset, t.length, t.first_segment), i
+        for i in list(range(0, len(segs), 31)) + [len(segs) - 1]:
+            m, sg = meta.segment(i), segs[i]
+            assert (m.seed, m.bit_width, m.payload_offset, m.payload_bytes) == (
+                sg.seed, sg.bit_width, sg.payload_offset, sg.payload_bytes), i
+        b = bytearray(data)
+        index_end = 64 + 32 * info.tile_count
+        positions = list(rng.integers(64, index_end, 8)) + list(rng.integers(index_end, len(data), 40))
+        positions += [len(data) - 1, len(data) - 5, index_end, index_end + 4]
+        for pos in positions:
+            pos = int(pos)
+            for v in {0, 0xFF, b[pos] ^ 0x40}:
+                if v == data[pos]:
+                    continue
+                b[pos] = v
+                assert ts.parse_metadata(bytes(b))[0] == orc.parse_status(bytes(b)), (pos, v)
+                b[pos] = data[pos]
+        for n in (len(data) - 1, len(data) // 2, index_end + 1):
+            assert ts.parse_metadata(data[:n])[0] == orc.parse_status(data[:n])
+
+
 @pytest.mark.parametrize("tile", [(128, 32), (64, 64), (256, 16)])
 def test_product_cpu_codec_matches_oracle(orc, tile):
     for w, h, seed in [(1024, 512, 1), (333, 517, 2), (1, 1, 3), (129, 33, 4)]:

@@ -10,7 +10,10 @@
 #include <algorithm>
 #include <new>
 #include <stdexcept>
+#include <system_error>
+#include <thread>
 #include <utility>
+#include <vector>
 
 namespace tilestream {
 namespace {
@@ -62,13 +65,14 @@ Status read_fixed_header(const std::uint8_t *d, std::uint64_t size, Geometry &g)
     return Status::success;
 }
 
-// One pass over the index and every payload.  With `out` null it only counts
-// predictive rows; otherwise it fills the preallocated tables.
-Status walk(const std::uint8_t *d, std::uint64_t size, const Geometry &g, std::uint64_t &rows,
-            ImageMetadata *out) noexcept {
+// The index alone: every entry's fields, payloads laid end to end in tile
+// order up to the end of the file, and the first row slot of each predictive
+// tile (a predictive tile has one row per tile row).  Every failure is
+// malformed_data, as in one sequential walk, so the passes may be split.
+Status walk_index(const std::uint8_t *d, std::uint64_t size, const Geometry &g, std::vector<std::uint64_t> &first,
+                  std::uint64_t &rows) noexcept {
     using host::load_le;
     const std::uint64_t s = g.image.type == PixelType::uint16 ? 2 : 4;
-    const std::uint32_t widest = g.image.type == PixelType::uint16 ? 16 : 32;
     std::uint64_t next = g.payload_at;
     rows = 0;
     for (std::uint64_t i = 0; i < g.count; ++i) {
@@ -78,15 +82,38 @@ Status walk(const std::uint8_t *d, std::uint64_t size, const Geometry &g, std::u
         const std::uint64_t h = std::min<std::uint64_t>(g.tile.height, g.image.height - y);
         const std::uint64_t at = load_le<std::uint64_t>(e + ent::offset);
         const std::uint64_t len = load_le<std::uint64_t>(e + ent::length);
-        const std::uint64_t raw = w * h * s, raw4 = (raw + 3) / 4 * 4;
+        const std::uint64_t raw4 = (w * h * s + 3) / 4 * 4;
         const std::uint8_t mode = e[ent::mode];
         if (mode > static_cast<std::uint8_t>(TileMode::predict_bitpack) || !host::zeros(e + ent::reserved, 3) ||
             !host::zeros(e + ent::reserved_tail, 8) || load_le<std::uint16_t>(e + ent::width) != w ||
             load_le<std::uint16_t>(e + ent::height) != h || at != next || at > size ||
             len > size - at || len == 0 || len % 4 != 0 || len > raw4)
             return Status::malformed_data;
+        first[static_cast<std::size_t>(i)] = rows;
+        if (mode == static_cast<std::uint8_t>(TileMode::predict_bitpack)) rows += h;
+        next += len;
+    }
+    return next == size ? Status::success : Status::malformed_data;
+}
+
+// Payloads of tiles [begin, end), whose index entries walk_index accepted.
+// With `out` null it only validates; otherwise it fills the preallocated tables.
+Status walk_payloads(const std::uint8_t *d, const Geometry &g, const std::uint64_t *first, std::uint64_t begin,
+                     std::uint64_t end, ImageMetadata *out) noexcept {
+    using host::load_le;
+    const std::uint64_t s = g.image.type == PixelType::uint16 ? 2 : 4;
+    const std::uint32_t widest = g.image.type == PixelType::uint16 ? 16 : 32;
+    for (std::uint64_t i = begin; i < end; ++i) {
+        const std::uint8_t *e = d + format::header_bytes + i * format::index_entry_bytes;
+        const std::uint64_t x = i % g.across * g.tile.width, y = i / g.across * g.tile.height;
+        const std::uint64_t w = std::min<std::uint64_t>(g.tile.width, g.image.width - x);
+        const std::uint64_t h = std::min<std::uint64_t>(g.tile.height, g.image.height - y);
+        const std::uint64_t at = load_le<std::uint64_t>(e + ent::offset);
+        const std::uint64_t len = load_le<std::uint64_t>(e + ent::length);
+        const std::uint64_t raw = w * h * s, raw4 = (raw + 3) / 4 * 4;
+        const std::uint8_t mode = e[ent::mode];
         const std::uint8_t *p = d + at;
-        std::uint64_t first = 0;
+        std::uint64_t row = 0;
         switch (static_cast<TileMode>(mode)) {
         case TileMode::raw:
             if (len != raw4 || !host::zeros(p + raw, len - raw)) return Status::malformed_data;
@@ -95,7 +122,7 @@ Status walk(const std::uint8_t *d, std::uint64_t size, const Geometry &g, std::u
             if (len != 4 || (s == 2 && !host::zeros(p + 2, 2))) return Status::malformed_data;
             break;
         case TileMode::predict_bitpack: {
-            first = rows;
+            row = first[static_cast<std::size_t>(i)];
             std::uint64_t pos = 0;
             for (std::uint64_t r = 0; r < h; ++r) {
                 if (len - pos < format::segment_header_bytes) return Status::malformed_data;
@@ -110,8 +137,7 @@ Status walk(const std::uint8_t *d, std::uint64_t size, const Geometry &g, std::u
                 // High bits of the last word beyond the final residual must be zero.
                 if (bits % 32 != 0 && (load_le<std::uint32_t>(p + pos + packed - 4) >> (bits % 32)) != 0)
                     return Status::malformed_data;
-                if (out != nullptr) out->segments[rows] = {seed, b, pos, packed};
-                ++rows;
+                if (out != nullptr) out->segments[static_cast<std::size_t>(row + r)] = {seed, b, pos, packed};
                 pos += packed;
             }
             if (pos != len) return Status::malformed_data;
@@ -119,11 +145,40 @@ Status walk(const std::uint8_t *d, std::uint64_t size, const Geometry &g, std::u
         }
         }
         if (out != nullptr)
-            out->tiles[i] = {static_cast<TileMode>(mode), static_cast<std::uint16_t>(w),
-                             static_cast<std::uint16_t>(h), at, len, first};
-        next += len;
+            out->tiles[static_cast<std::size_t>(i)] = {static_cast<TileMode>(mode), static_cast<std::uint16_t>(w),
+                                                       static_cast<std::uint16_t>(h), at, len, row};
     }
-    return next == size ? Status::success : Status::malformed_data;
+    return Status::success;
+}
+
+// walk_payloads over every tile; archives of a few MiB and more are split
+// into contiguous tile ranges over host threads (the ranges touch disjoint
+// payloads and table slots).
+Status walk_all(const std::uint8_t *d, std::uint64_t size, const Geometry &g, const std::uint64_t *first,
+                ImageMetadata *out) noexcept {
+    constexpr std::uint64_t kBytesPerThread = 4ull << 20;
+    const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+    const std::uint64_t threads =
+        std::min<std::uint64_t>({std::uint64_t{hw}, 16, size / kBytesPerThread + 1, (g.count + 63) / 64});
+    if (threads <= 1) return walk_payloads(d, g, first, 0, g.count, out);
+    try {
+        std::vector<Status> status(static_cast<std::size_t>(threads), Status::success);
+        std::vector<std::thread> pool;
+        pool.reserve(static_cast<std::size_t>(threads - 1));
+        const std::uint64_t per = (g.count + threads - 1) / threads;
+        for (std::uint64_t k = 1; k < threads; ++k)
+            pool.emplace_back([&, k] {
+                status[static_cast<std::size_t>(k)] =
+                    walk_payloads(d, g, first, std::min(g.count, k * per), std::min(g.count, (k + 1) * per), out);
+            });
+        status[0] = walk_payloads(d, g, first, 0, std::min(g.count, per), out);
+        for (std::thread &t : pool) t.join();
+        for (const Status st : status)
+            if (st != Status::success) return st;
+        return Status::success;
+    } catch (const std::system_error &) {
+        return walk_payloads(d, g, first, 0, g.count, out); // no threads available: one pass here
+    }
 }
 
 Status native_bytes(std::uint64_t tiles, std::uint64_t segments, std::uint64_t &bytes) noexcept {
@@ -146,18 +201,21 @@ Status parse_metadata(ConstBytes source, const ParseLimits &limits, ImageMetadat
     std::uint64_t need = 0;
     if (const Status st = native_bytes(g.count, 0, need); st != Status::success) return st;
     if (need > limits.metadata_bytes) return Status::resource_limit;
-    std::uint64_t rows = 0;
-    if (const Status st = walk(d, source.size, g, rows, nullptr); st != Status::success) return st;
-    if (const Status st = native_bytes(g.count, rows, need); st != Status::success) return st;
-    if (need > limits.metadata_bytes) return Status::resource_limit;
     try {
+        // The index is at most 1/32 of the file, so its row table is small.
+        std::vector<std::uint64_t> first(static_cast<std::size_t>(g.count));
+        std::uint64_t rows = 0;
+        if (const Status st = walk_index(d, source.size, g, first, rows); st != Status::success) return st;
+        if (const Status st = walk_all(d, source.size, g, first.data(), nullptr); st != Status::success) return st;
+        if (const Status st = native_bytes(g.count, rows, need); st != Status::success) return st;
+        if (need > limits.metadata_bytes) return Status::resource_limit;
         ImageMetadata parsed;
         parsed.image = g.image;
         parsed.tile = g.tile;
         parsed.file_bytes = source.size;
         parsed.tiles.resize(static_cast<std::size_t>(g.count));
         parsed.segments.resize(static_cast<std::size_t>(rows));
-        if (const Status st = walk(d, source.size, g, rows, &parsed); st != Status::success) return st;
+        if (const Status st = walk_all(d, source.size, g, first.data(), &parsed); st != Status::success) return st;
         metadata = std::move(parsed);
         return Status::success;
     } catch (const std::bad_alloc &) {
