@@ -289,7 +289,14 @@ struct Server::Impl {
         return layout_for(n, n, options.max_tasks, n + 2 * (options.max_tasks / options.encode_window + 2));
     }
     Status extent(const ServeRequest &r, Extent &e) const noexcept;
-    Status launch(Slot &slot, const ServeRequest *requests, std::uint64_t first, std::uint64_t count) noexcept;
+    // Waits for every batch in flight and frees the slots.
+    void drain() noexcept {
+        for (Slot &slot : slots)
+            if (slot.busy) static_cast<void>(hipStreamSynchronize(slot.stream));
+        for (Slot &slot : slots) slot.busy = false;
+    }
+    // Allocates its window tables: may throw std::bad_alloc (replay turns it into resource_limit).
+    Status launch(Slot &slot, const ServeRequest *requests, std::uint64_t first, std::uint64_t count);
 };
 
 Server::Server(std::unique_ptr<Impl> impl) noexcept : impl_(std::move(impl)) {}
@@ -416,7 +423,7 @@ Status Server::check(const ServeRequest &request) const noexcept {
 // Stages the batch's tables, copies them to the device with one kernel, runs
 // the region decode in request mode and the encode windows, and copies the
 // final results into pinned memory; then marks the end.
-Status Server::Impl::launch(Slot &slot, const ServeRequest *requests, std::uint64_t first, std::uint64_t count) noexcept {
+Status Server::Impl::launch(Slot &slot, const ServeRequest *requests, std::uint64_t first, std::uint64_t count) {
     std::uint64_t regions = 0, tasks = 0;
     std::uint64_t encodes[2] = {0, 0}; // uint16, float32
     std::vector<std::uint64_t> enc_tiles[2];
@@ -721,17 +728,17 @@ Status Server::replay(const ServeRequest *requests, const std::uint64_t *arrival
             }
         }
         if (failure != Status::success) {
-            for (Slot &slot : s.slots)
-                if (slot.busy) static_cast<void>(hipStreamSynchronize(slot.stream));
-            for (Slot &slot : s.slots) slot.busy = false;
+            s.drain();
             return failure;
         }
         stats.requests = count;
         stats.seconds = static_cast<double>(last_done - std::min(last_done, arrival_ns[0])) * 1e-9;
         return Status::success;
     } catch (const std::bad_alloc &) {
+        s.drain();
         return Status::resource_limit;
     } catch (const std::length_error &) {
+        s.drain();
         return Status::resource_limit;
     }
 }
