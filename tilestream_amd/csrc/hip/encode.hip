@@ -33,11 +33,22 @@
 #include <tilestream_kernels.h>
 
 #include <cstdint>
+#include <type_traits>
 
 namespace {
 
-constexpr std::uint32_t kThreads = 256;
-constexpr std::uint32_t kMaxTileWords = 4096 + 64 * 4; // rows of tile_w + lanes-per-row words
+// Per pixel type: threads per workgroup (one tile each) and the LDS sample.
+// uint16 tiles keep 16-bit samples and use 128 threads, so twice as many
+// tiles fit a CU; each lane holds 4096 / threads columns of one tile row.
+template <bool U16> struct Cfg {
+    static constexpr std::uint32_t threads = U16 ? 128u : 256u;
+    static constexpr std::uint32_t columns = 4096u / threads; // per lane
+    // Waves per SIMD the registers are budgeted for: 14 uint16 or 8 float32 tiles per CU.
+    static constexpr std::uint32_t waves = U16 ? 7u : 8u;
+    using sample = std::conditional_t<U16, std::uint16_t, std::uint32_t>;
+    // Rows of tile_w + lanes samples (lanes x tile_h = threads), as 32-bit words.
+    static constexpr std::uint32_t words = (4096u + threads) * sizeof(sample) / 4u;
+};
 constexpr std::uint64_t kFlagAggregate = 1ull << 62;    // value: this tile's length
 constexpr std::uint64_t kFlagPrefix = 2ull << 62;       // value: lengths from the image's first tile here through this one
 constexpr std::uint64_t kFlagMask = 3ull << 62;
@@ -119,7 +130,9 @@ __device__ inline std::uint64_t wave_sum(std::uint64_t v) {
 // sample.  Every load of a thread is issued before the first one is waited
 // for (a tile is at most 4 chunks of 16 bytes, or 16 samples, per thread).
 template <bool U16>
-__device__ inline void stage_tile(const tsk_encode_image &img, const Geometry &g, std::uint32_t stride, std::uint32_t *px) {
+__device__ inline void stage_tile(const tsk_encode_image &img, const Geometry &g, std::uint32_t stride,
+                                  typename Cfg<U16>::sample *px) {
+    constexpr std::uint32_t kThreads = Cfg<U16>::threads;
     constexpr std::uint32_t s = U16 ? 2u : 4u;
     const std::uint8_t *origin = img.in + g.y * img.in_pitch + g.x * s;
     const bool vector = ((reinterpret_cast<std::uintptr_t>(img.in) | img.in_pitch) & 15u) == 0;
@@ -145,11 +158,15 @@ __device__ inline void stage_tile(const tsk_encode_image &img, const Geometry &g
             const std::uint32_t k = threadIdx.x + q * kThreads;
             if (k >= chunks) continue;
             const std::uint32_t r = div_magic(k, magic, across), c = k - r * across;
-            std::uint32_t *dst = px + r * stride + c * per;
+            typename Cfg<U16>::sample *dst = px + r * stride + c * per;
             if ((c + 1u) * per <= g.w) {
                 if constexpr (U16) {
-                    reinterpret_cast<u32x4 *>(dst)[0] = u32x4{v[q].x & 0xffffu, v[q].x >> 16, v[q].y & 0xffffu, v[q].y >> 16};
-                    reinterpret_cast<u32x4 *>(dst)[1] = u32x4{v[q].z & 0xffffu, v[q].z >> 16, v[q].w & 0xffffu, v[q].w >> 16};
+                    // 8 samples at a 4-byte boundary (the stride is even).
+                    std::uint32_t *d32 = reinterpret_cast<std::uint32_t *>(dst);
+                    d32[0] = v[q].x;
+                    d32[1] = v[q].y;
+                    d32[2] = v[q].z;
+                    d32[3] = v[q].w;
                 } else {
                     reinterpret_cast<u32x4 *>(dst)[0] = v[q];
                 }
@@ -162,31 +179,39 @@ __device__ inline void stage_tile(const tsk_encode_image &img, const Geometry &g
             }
         }
     } else {
-        constexpr std::uint32_t most = 4096u / kThreads; // samples per thread
+        // Rounds of 16 samples per thread, all loads of a round issued first.
+        constexpr std::uint32_t round = 16;
         const std::uint32_t n = g.w * g.h;
         const std::uint32_t magic = magic_of(g.w);
-        std::uint32_t v[most];
+        for (std::uint32_t k0 = 0; k0 < 4096u; k0 += round * kThreads) {
+            std::uint32_t v[round];
 #pragma unroll
-        for (std::uint32_t q = 0; q < most; ++q) {
-            const std::uint32_t k = threadIdx.x + q * kThreads;
-            const std::uint32_t r = div_magic(k, magic, g.w), c = k - r * g.w;
-            const std::uint8_t *p = origin + std::uint64_t{r} * img.in_pitch;
-            if (k < n)
-                v[q] = U16 ? reinterpret_cast<gptr<const std::uint16_t>>(global(p))[c]
-                           : reinterpret_cast<gptr<const std::uint32_t>>(global(p))[c];
-        }
+            for (std::uint32_t q = 0; q < round; ++q) {
+                const std::uint32_t k = k0 + threadIdx.x + q * kThreads;
+                const std::uint32_t r = div_magic(k, magic, g.w), c = k - r * g.w;
+                const std::uint8_t *p = origin + std::uint64_t{r} * img.in_pitch;
+                if (k < n)
+                    v[q] = U16 ? reinterpret_cast<gptr<const std::uint16_t>>(global(p))[c]
+                               : reinterpret_cast<gptr<const std::uint32_t>>(global(p))[c];
+            }
 #pragma unroll
-        for (std::uint32_t q = 0; q < most; ++q) {
-            const std::uint32_t k = threadIdx.x + q * kThreads;
-            const std::uint32_t r = div_magic(k, magic, g.w), c = k - r * g.w;
-            if (k < n) px[r * stride + c] = v[q];
+            for (std::uint32_t q = 0; q < round; ++q) {
+                const std::uint32_t k = k0 + threadIdx.x + q * kThreads;
+                const std::uint32_t r = div_magic(k, magic, g.w), c = k - r * g.w;
+                if (k < n) px[r * stride + c] = static_cast<typename Cfg<U16>::sample>(v[q]);
+            }
         }
     }
 }
 
 template <bool U16>
-__global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(8))) encode_kernel(tsk_encode_args a) {
-    __shared__ __attribute__((aligned(16))) std::uint32_t px[kMaxTileWords];
+__global__ void __launch_bounds__(Cfg<U16>::threads) __attribute__((amdgpu_waves_per_eu(Cfg<U16>::waves))) encode_kernel(tsk_encode_args a) {
+    constexpr std::uint32_t kThreads = Cfg<U16>::threads;
+    constexpr std::uint32_t kColumns = Cfg<U16>::columns;
+    using Sample = typename Cfg<U16>::sample;
+    // The tile's samples; after the analysis of a predictive tile, its payload words.
+    __shared__ __attribute__((aligned(16))) std::uint32_t lds[Cfg<U16>::words];
+    Sample *px = reinterpret_cast<Sample *>(lds);
     __shared__ std::uint32_t row_word[65];
     __shared__ std::uint32_t row_bits[64];
     __shared__ std::uint32_t wave_cost[kThreads / 64];
@@ -204,9 +229,9 @@ __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(8
     const tsk_encode_image &img = a.images[e.image];
     const std::uint64_t index = e.start + (t - e.first);
     const Geometry g = geometry(img, index);
-    // Lanes per tile row: 4, 8 or 16 for 64, 32 or 16 rows; each lane holds
-    // 16 columns (tile_w / lanes) of its row.  Row r sits at px + r * stride,
-    // with stride = tile_w + lanes so that a wave's reads fall in distinct banks.
+    // Lanes per tile row: threads / tile_h; each lane holds kColumns columns
+    // (tile_w / lanes) of its row.  Row r sits at px + r * stride, with stride
+    // = tile_w + lanes samples so that a wave's reads fall in distinct banks.
     const std::uint32_t lanes = kThreads / img.tile_h;
     const std::uint32_t stride = img.tile_w + lanes;
     const std::uint32_t r = threadIdx.x / lanes, sl = threadIdx.x % lanes;
@@ -218,18 +243,23 @@ __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(8
     const std::uint32_t first = px[0];
     bool same = true;
     std::uint32_t any = 0, seed = 0;
-    std::uint32_t z[16];
+    // Residuals stay in registers until the packing; uint16 residuals (at
+    // most 16 bits) two to a register.
+    constexpr std::uint32_t kPer = U16 ? 2u : 1u;
+    std::uint32_t z[kColumns / kPer];
 #pragma unroll
-    for (std::uint32_t i = 0; i < 16; ++i) {
+    for (std::uint32_t i = 0; i < kColumns / kPer; ++i) z[i] = 0;
+#pragma unroll
+    for (std::uint32_t i = 0; i < kColumns; ++i) {
         const std::uint32_t c = sl + lanes * i;
-        z[i] = 0;
         if (r < g.h && c < g.w) {
             const std::uint32_t v = px[r * stride + c];
             same = same && v == first;
             if (c == 0) seed = v;
             if (c != 0) {
-                z[i] = residual<U16>(v, px[r * stride + c - 1u]);
-                any |= z[i];
+                const std::uint32_t zi = residual<U16>(v, px[r * stride + c - 1u]);
+                z[i / kPer] |= zi << (U16 ? 16u * (i % 2u) : 0u);
+                any |= zi;
             }
         }
     }
@@ -280,24 +310,25 @@ __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(8
         // The samples were last read before the barrier above: their area
         // becomes the payload, zeroed first.
         for (std::uint32_t k = threadIdx.x * 4u; k < length / 4u; k += kThreads * 4u)
-            *reinterpret_cast<uint4 *>(px + k) = make_uint4(0u, 0u, 0u, 0u);
+            *reinterpret_cast<u32x4 *>(lds + k) = u32x4{0u, 0u, 0u, 0u};
         __syncthreads();
         if (r < g.h) {
             // Each lane ORs its residuals (still in registers) into the row's
             // LSB-first bit stream: residual of column c at bit (c - 1) * b.
             const std::uint32_t b = row_bits[r], base = row_word[r];
             if (sl == 0) {
-                px[base] = seed;
-                px[base + 1u] = b;
+                lds[base] = seed;
+                lds[base + 1u] = b;
             }
 #pragma unroll
-            for (std::uint32_t i = 0; i < 16; ++i) {
+            for (std::uint32_t i = 0; i < kColumns; ++i) {
                 const std::uint32_t c = sl + lanes * i;
-                if (z[i] != 0 && c < g.w) {
+                const std::uint32_t zi = U16 ? (z[i / kPer] >> (16u * (i % 2u))) & 0xffffu : z[i];
+                if (zi != 0 && c < g.w) {
                     const std::uint32_t pos = (c - 1u) * b;
                     const std::uint32_t w = base + 2u + (pos >> 5), sh = pos & 31u;
-                    atomicOr(&px[w], z[i] << sh);
-                    if (sh + b > 32u) atomicOr(&px[w + 1u], z[i] >> (32u - sh));
+                    atomicOr(&lds[w], zi << sh);
+                    if (sh + b > 32u) atomicOr(&lds[w + 1u], zi >> (32u - sh));
                 }
             }
         }
@@ -362,8 +393,8 @@ __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(8
         // 16-byte stores at the payload's word alignment, then the last words.
         const std::uint32_t quads = words / 4u;
         for (std::uint32_t k = threadIdx.x; k < quads; k += kThreads)
-            *reinterpret_cast<gptr<u32x4w>>(dst + 4u * k) = *reinterpret_cast<const u32x4 *>(px + 4u * k);
-        if (threadIdx.x < words - 4u * quads) dst[4u * quads + threadIdx.x] = px[4u * quads + threadIdx.x];
+            *reinterpret_cast<gptr<u32x4w>>(dst + 4u * k) = *reinterpret_cast<const u32x4 *>(lds + 4u * k);
+        if (threadIdx.x < words - 4u * quads) dst[4u * quads + threadIdx.x] = lds[4u * quads + threadIdx.x];
     } else if (mode == 0) {
         const std::uint32_t magic = magic_of(g.w);
         const std::uint32_t n = g.w * g.h;
@@ -374,7 +405,7 @@ __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(8
                 v = px[r0 * stride + (p0 - r0 * g.w)];
                 if (p0 + 1u < n) {
                     const std::uint32_t r1 = div_magic(p0 + 1u, magic, g.w);
-                    v |= px[r1 * stride + (p0 + 1u - r1 * g.w)] << 16;
+                    v |= std::uint32_t{px[r1 * stride + (p0 + 1u - r1 * g.w)]} << 16;
                 }
             } else {
                 const std::uint32_t r0 = div_magic(k, magic, g.w);
@@ -441,9 +472,11 @@ int tsk_encode_reset(const tsk_encode_args *a, void *stream) {
 int tsk_encode_window(const tsk_encode_args *a, void *stream) {
     if (a->tile_count == 0) return hipSuccess;
     if (a->type == TSK_UINT16)
-        hipLaunchKernelGGL(encode_kernel<true>, dim3(a->tile_count), dim3(kThreads), 0, static_cast<hipStream_t>(stream), *a);
+        hipLaunchKernelGGL(encode_kernel<true>, dim3(a->tile_count), dim3(Cfg<true>::threads), 0,
+                           static_cast<hipStream_t>(stream), *a);
     else
-        hipLaunchKernelGGL(encode_kernel<false>, dim3(a->tile_count), dim3(kThreads), 0, static_cast<hipStream_t>(stream), *a);
+        hipLaunchKernelGGL(encode_kernel<false>, dim3(a->tile_count), dim3(Cfg<false>::threads), 0,
+                           static_cast<hipStream_t>(stream), *a);
     return hipGetLastError();
 }
 
