@@ -107,6 +107,20 @@ def profiled_traffic(name="decode_batch256_kernel_trace.md", key="traffic_bytes_
     return None
 
 
+def copy_ceiling(stream, total_bytes: int, steps: int = 5):
+    """HBM rate of a plain streaming copy moving `total_bytes` (half read, half
+    written) in the fastest copy shape measured here (tsk_stream_copy): the
+    practical ceiling for a kernel that reads and writes about as much."""
+    from tilestream_amd import api
+    from tilestream_amd import _native as N
+    half = (total_bytes // 2) // 4096 * 4096
+    a, b = api.DeviceMemory(half), api.DeviceMemory(half)
+    fn = lambda: N.lib().tsk_stream_copy(b.ptr, a.ptr, half // 16, stream.handle)  # noqa: E731
+    fn()
+    ms = timed(stream, fn, steps)
+    return 2 * half / (ms * 1e-3) / 1e9
+
+
 def percentile(values, q: float) -> float:
     """Nearest-rank percentile (the reference harness's convention)."""
     v = np.sort(np.asarray(values))
@@ -1058,6 +1072,8 @@ def main() -> int:
         verified = ok
         log(f"verification bit-exact: {ok}")
 
+    ceiling = copy_ceiling(stream, stats.payload_bytes + stats.output_bytes + stats.map_bytes)
+    log(f"copy ceiling for the same bytes: {ceiling:.1f} GB/s")
     cpu = None if args.no_cpu else cpu_baseline(batch, threads, 1_000_000_000)
     del plans, pws, rec
     roi = None if args.no_roi else roi_leg(args, threads, stream)
@@ -1104,10 +1120,14 @@ def main() -> int:
             "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": traffic,
             "algorithmic_bytes_per_launch": traffic_alg,
+            "copy_ceiling": round(ceiling, 1),
+            "frac_of_copy_ceiling": round(achieved / ceiling, 4),
             "note": "payload bytes of every tile read + decoded output bytes written, over the "
                     "graph launch time (results init, counter reset, decode, results completion; the "
                     "decode kernel is 98-99% of it); traffic = HBM bytes per launch from rocprofv3 "
-                    "FETCH_SIZE/WRITE_SIZE on this workload (profiles/decode_batch256_kernel_trace.md)",
+                    "FETCH_SIZE/WRITE_SIZE on this workload (profiles/decode_batch256_kernel_trace.md); "
+                    "copy_ceiling = a plain 16-byte streaming copy moving the same number of bytes, "
+                    "measured in this run",
         },
         "autotune": {"group": best.group, "block": best.block, "chunk": best.chunk, "depth": best.depth,
                      "candidates": len(candidates)},

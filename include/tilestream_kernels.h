@@ -198,6 +198,9 @@ int tsk_preload(uint32_t op);
 /* Copies `blocks` 16-byte blocks with a plain streaming kernel (the known-size
  * reference for scaling hardware byte counters). */
 int tsk_copy_blocks(void *dst, const void *src, uint64_t blocks, void *stream);
+/* Copies `blocks` 16-byte blocks in the fastest plain streaming shape measured
+ * on this hardware: the practical ceiling for kernels that read and write. */
+int tsk_stream_copy(void *dst, const void *src, uint64_t blocks, void *stream);
 /* Zero-fills `words` 32-bit words (a kernel, safe under graph capture). */
 int tsk_fill_u32(uint32_t *data, uint64_t words, uint32_t value, void *stream);
 

@@ -72,6 +72,29 @@ __global__ void copy_kernel(uint4 *__restrict__ dst, const uint4 *__restrict__ s
         dst[i] = src[i];
 }
 
+// The fastest plain copy shape measured on this hardware (four 16-byte blocks
+// in flight per thread, non-temporal, 32768 workgroups): the practical
+// ceiling that a kernel reading and writing about as much is compared with.
+__global__ void __launch_bounds__(256) stream_copy_kernel(tsk::u32x4 *__restrict__ dst,
+                                                          const tsk::u32x4 *__restrict__ src, std::uint64_t blocks) {
+    constexpr int kUnroll = 4;
+    const std::uint64_t stride = std::uint64_t{gridDim.x} * blockDim.x * kUnroll;
+    for (std::uint64_t base = std::uint64_t{blockIdx.x} * blockDim.x * kUnroll + threadIdx.x; base < blocks;
+         base += stride) {
+        tsk::u32x4 v[kUnroll];
+#pragma unroll
+        for (int u = 0; u < kUnroll; ++u) {
+            const std::uint64_t i = base + std::uint64_t{static_cast<unsigned>(u)} * blockDim.x;
+            if (i < blocks) v[u] = __builtin_nontemporal_load(src + i);
+        }
+#pragma unroll
+        for (int u = 0; u < kUnroll; ++u) {
+            const std::uint64_t i = base + std::uint64_t{static_cast<unsigned>(u)} * blockDim.x;
+            if (i < blocks) __builtin_nontemporal_store(v[u], dst + i);
+        }
+    }
+}
+
 __global__ void fill_kernel(std::uint32_t *data, std::uint64_t words, std::uint32_t value) {
     const std::uint64_t stride = std::uint64_t{gridDim.x} * blockDim.x;
     for (std::uint64_t i = std::uint64_t{blockIdx.x} * blockDim.x + threadIdx.x; i < words; i += stride)
@@ -149,6 +172,15 @@ int tsk_copy_blocks(void *dst, const void *src, std::uint64_t blocks, void *stre
     hipLaunchKernelGGL(copy_kernel, dim3(groups < 4096 ? static_cast<unsigned>(groups) : 4096u), dim3(256), 0,
                        static_cast<hipStream_t>(stream),
                        static_cast<uint4 *>(dst), static_cast<const uint4 *>(src), blocks);
+    return hipGetLastError();
+}
+
+int tsk_stream_copy(void *dst, const void *src, std::uint64_t blocks, void *stream) {
+    if (blocks == 0) return hipSuccess;
+    const std::uint64_t groups = (blocks + 1023) / 1024;
+    hipLaunchKernelGGL(stream_copy_kernel, dim3(groups < 32768 ? static_cast<unsigned>(groups) : 32768u), dim3(256),
+                       0, static_cast<hipStream_t>(stream), static_cast<tsk::u32x4 *>(dst),
+                       static_cast<const tsk::u32x4 *>(src), blocks);
     return hipGetLastError();
 }
 
