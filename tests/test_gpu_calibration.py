@@ -5,6 +5,9 @@ Tolerance: FP32 results equal the host binary32 computation bit for bit
 reference the error stays below 1.5 ulp for normal results and below 2^-149
 absolute for subnormal ones (reference docs/numerical-analysis.md "Error bounds").
 """
+import json
+import os
+
 import numpy as np
 import pytest
 
@@ -26,6 +29,17 @@ def same_or_both_nan(a, b):
     fa, fb = a.view(np.float32), b.view(np.float32)
     both_nan = np.isnan(fa) & np.isnan(fb)
     return bool(np.all((a == b) | both_nan)), int(np.sum(~((a == b) | both_nan)))
+
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def contract_rows():
+    """The reference's 42 calibration contract rows (tests/golden/calibration_contract.json):
+    float32 sample words, offsets, gains and the exact expected words, as a 1 x 42 image."""
+    rows = json.load(open(os.path.join(HERE, "golden", "calibration_contract.json")))["rows"]
+    col = lambda k: np.array([r[k] for r in rows], np.uint32).reshape(1, len(rows))  # noqa: E731
+    return col("x"), col("offset").view(np.float32), col("gain").view(np.float32), col("expected")
 
 
 def calibrate_on_gpu(samples, offset, gain):
@@ -131,3 +145,29 @@ def test_fused_scalar_region(orc, tile):
         assert status[0] == 0
         want = orc.decode_roi_calibrate(data, (x, y, rw, rh), 100.0, 1.05).view(np.uint32)
         assert np.array_equal(outs[0], want)
+
+
+def test_contract_rows_standalone_with_maps():
+    x, o, gn, want = contract_rows()
+    ok, bad = same_or_both_nan(calibrate_on_gpu(x, o, gn), want)
+    assert ok, bad
+
+
+def test_contract_rows_standalone_with_scalars():
+    x, o, gn, want = contract_rows()
+    for i in range(x.shape[1]):
+        got = calibrate_on_gpu(np.ascontiguousarray(x[:, i:i + 1]), float(o[0, i]), float(gn[0, i]))
+        ok, _ = same_or_both_nan(got, want[:, i:i + 1])
+        assert ok, (i, hex(int(got[0, 0])), hex(int(want[0, i])))
+
+
+def test_contract_rows_fused_with_decode(orc):
+    """The same rows as float32 samples of an archive, decoded fused with maps."""
+    x, o, gn, want = contract_rows()
+    data = orc.compress(x, (64, 64))
+    job = Job()
+    job.add_decode(data, calibrated=True, offset=o, gain=gn)
+    status, written, outs = job.run(ts.Operation.decode_calibrate)
+    assert status[0] == 0 and written[0] == x.size * 4
+    ok, bad = same_or_both_nan(outs[0], want)
+    assert ok, bad

@@ -222,3 +222,17 @@ def test_calibration_host_form(orc):
     r = orc.calibrate(f.view(np.uint32), np.float32(-2.0**-24), np.float32(3.0))
     assert r[0, 0] == np.float32(3.0)
     del x
+
+
+def test_calibration_contract_rows(orc):
+    """The oracle's host binary32 calibration gives the reference's 42 contract
+    words (tests/golden/calibration_contract.json), any NaN for an expected NaN."""
+    rows = json.load(open(os.path.join(HERE, "golden", "calibration_contract.json")))["rows"]
+    x = np.array([r["x"] for r in rows], np.uint32).reshape(1, -1)
+    o = np.array([r["offset"] for r in rows], np.uint32).view(np.float32).reshape(1, -1)
+    g = np.array([r["gain"] for r in rows], np.uint32).view(np.float32).reshape(1, -1)
+    want = np.array([r["expected"] for r in rows], np.uint32).reshape(1, -1)
+    got = orc.calibrate(x, o, g).view(np.uint32)
+    nan = lambda w: ((w & 0x7F800000) == 0x7F800000) & ((w & 0x7FFFFF) != 0)  # noqa: E731
+    assert bool(np.all((got == want) | (nan(got) & nan(want)))), [r["note"] for r, a, b in zip(rows, got[0], want[0])
+                                                                  if a != b and not (nan(a) and nan(b))]
