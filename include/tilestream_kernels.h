@@ -169,7 +169,9 @@ typedef struct {
     uint32_t type;
 } tsk_calibrate_args;
 
-int tsk_results_begin(void *results, uint32_t count, void *stream);
+/* Zeroes `count` result records and, when `counter` is not NULL, `words`
+ * (at most 256) work-counter words, in one kernel. */
+int tsk_results_begin(void *results, uint32_t count, uint32_t *counter, uint32_t words, void *stream);
 int tsk_results_complete(const tsk_decode_image *images, uint32_t count, void *results,
                          void *stream);
 /* tsk_results_complete that writes the final records to `out` (for example

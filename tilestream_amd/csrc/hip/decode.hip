@@ -44,10 +44,13 @@ __global__ void __launch_bounds__(256) stage_kernel(tsk_decode_args a) {
     }
 }
 
-__global__ void results_begin_kernel(ResultBits *results, std::uint32_t count) {
+// Zeroes the result records and, when given, the work counter words.
+__global__ void results_begin_kernel(ResultBits *results, std::uint32_t count, std::uint32_t *counter,
+                                     std::uint32_t words) {
     const std::uint64_t stride = std::uint64_t{gridDim.x} * blockDim.x;
     for (std::uint64_t i = std::uint64_t{blockIdx.x} * blockDim.x + threadIdx.x; i < count; i += stride)
         results[i] = {0u, 0u, 0u};
+    if (counter != nullptr && blockIdx.x == 0 && threadIdx.x < words) counter[threadIdx.x] = 0;
 }
 
 __global__ void results_complete_kernel(const tsk_decode_image *images, std::uint32_t count, const ResultBits *results,
@@ -122,10 +125,12 @@ std::uint32_t small_grid(std::uint64_t n) {
 
 extern "C" {
 
-int tsk_results_begin(void *results, std::uint32_t count, void *stream) {
-    if (count == 0) return hipSuccess;
+int tsk_results_begin(void *results, std::uint32_t count, std::uint32_t *counter, std::uint32_t words,
+                      void *stream) {
+    if (count == 0 && counter == nullptr) return hipSuccess;
+    if (words > 256) return hipErrorInvalidValue;
     hipLaunchKernelGGL(results_begin_kernel, dim3(small_grid(count)), dim3(256), 0,
-                       static_cast<hipStream_t>(stream), static_cast<ResultBits *>(results), count);
+                       static_cast<hipStream_t>(stream), static_cast<ResultBits *>(results), count, counter, words);
     return hipGetLastError();
 }
 

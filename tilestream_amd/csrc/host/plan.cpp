@@ -600,7 +600,7 @@ Status check_execution(const PlanState *plan, Operation operation, MutableBytes 
 
 Status run_decode(const PlanState &plan, DeviceResult *results, hipStream_t stream, const DecodeShape &shape) noexcept {
     const auto count = static_cast<std::uint32_t>(plan.count);
-    if (tsk_results_begin(results, count, stream) != hipSuccess) return hip_failure();
+    if (tsk_results_begin(results, count, nullptr, 0, stream) != hipSuccess) return hip_failure();
     tsk_decode_args a{};
     a.images = plan.decode_images;
     a.tasks = plan.tasks;

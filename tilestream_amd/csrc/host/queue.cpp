@@ -163,8 +163,9 @@ Status DecodeQueue::run(MutableBytes workspace, ResultBuffer results, hipStream_
     }
     if (!tsk_decode_supported(sh.group, sh.block, sh.depth)) return Status::invalid_argument;
     auto *counter = static_cast<std::uint64_t *>(workspace.data);
-    if (tsk_results_begin(results.data, static_cast<std::uint32_t>(q.count), stream) != hipSuccess ||
-        (sh.chunk != 0 && tsk_fill_u32(static_cast<std::uint32_t *>(workspace.data), 2, 0, stream) != hipSuccess))
+    // One kernel zeroes the results and the work counter.
+    if (tsk_results_begin(results.data, static_cast<std::uint32_t>(q.count),
+                          sh.chunk != 0 ? static_cast<std::uint32_t *>(workspace.data) : nullptr, 2, stream) != hipSuccess)
         return detail::hip_failure();
     tsk_decode_args a{};
     a.images = q.images;
