@@ -93,3 +93,22 @@ def test_pending_caller_error_blocks_enqueue(orc):
     assert d.enqueue(None) == ts.Status.success
     assert api.synchronize(None) == ts.Status.success
     d.check()
+
+
+def test_enqueue_returns_before_the_work_completes(orc):
+    """Execution never waits for the device (reference tests/cuda_api_test.cpp:263-333):
+    enqueueing milliseconds of decode work returns long before it completes."""
+    import time
+    s = api.Stream()
+    d = Decode(orc, g.field_f32(8192, 4096, 96), (64, 64), s)  # 128 MiB decoded per call
+    assert d.enqueue(s) == ts.Status.success  # first run outside the timing
+    s.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(100):
+        assert d.enqueue(s) == ts.Status.success
+    t1 = time.perf_counter()
+    s.synchronize()
+    t2 = time.perf_counter()
+    assert t2 - t0 > 0.002, "too little work to tell enqueue from completion"
+    assert t1 - t0 < 0.5 * (t2 - t0), (t1 - t0, t2 - t0)
+    d.check()
