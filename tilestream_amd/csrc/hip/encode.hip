@@ -42,13 +42,22 @@ namespace {
 // tiles fit a CU; each lane holds 4096 / threads columns of one tile row.
 template <bool U16> struct Cfg {
     static constexpr std::uint32_t threads = U16 ? 128u : 256u;
-    static constexpr std::uint32_t columns = 4096u / threads; // per lane
+    // Each lane owns `columns` consecutive columns of one tile row.
+    static constexpr std::uint32_t columns = 4096u / threads;
     // Waves per SIMD the registers are budgeted for: 14 uint16 or 8 float32 tiles per CU.
     static constexpr std::uint32_t waves = U16 ? 7u : 8u;
     using sample = std::conditional_t<U16, std::uint16_t, std::uint32_t>;
-    // Rows of tile_w + lanes samples (lanes x tile_h = threads), as 32-bit words.
-    static constexpr std::uint32_t words = (4096u + threads) * sizeof(sample) / 4u;
+    // In LDS a lane's columns are a segment of columns + pad samples and a row
+    // is `lanes` segments: every lane of a wave then reads a distinct bank at
+    // each step (pad 1 for 32-bit, 2 for 16-bit samples).
+    static constexpr std::uint32_t segment = columns + (U16 ? 2u : 1u);
+    static constexpr std::uint32_t words = threads * segment * sizeof(sample) / 4u;
 };
+
+// LDS place of sample (r, c): row stride = lanes x segment samples.
+template <bool U16> __device__ inline std::uint32_t place(std::uint32_t r, std::uint32_t c, std::uint32_t row) {
+    return r * row + (c / Cfg<U16>::columns) * Cfg<U16>::segment + c % Cfg<U16>::columns;
+}
 constexpr std::uint64_t kFlagAggregate = 1ull << 62;    // value: this tile's length
 constexpr std::uint64_t kFlagPrefix = 2ull << 62;       // value: lengths from the image's first tile here through this one
 constexpr std::uint64_t kFlagMask = 3ull << 62;
@@ -78,14 +87,21 @@ struct Geometry {
 
 __device__ inline Geometry geometry(const tsk_encode_image &img, std::uint64_t tile) {
     Geometry g;
-    g.x = tile % img.tile_columns * img.tile_w;
-    g.y = tile / img.tile_columns * img.tile_h;
+    std::uint64_t row;
+    if (tile >> 32 == 0) {
+        row = static_cast<std::uint32_t>(tile) / img.tile_columns; // a 32-bit division where it suffices
+    } else {
+        row = tile / img.tile_columns;
+    }
+    g.x = (tile - row * img.tile_columns) * img.tile_w;
+    g.y = row * img.tile_h;
     g.w = static_cast<std::uint32_t>(img.width - g.x < img.tile_w ? img.width - g.x : img.tile_w);
     g.h = static_cast<std::uint32_t>(img.height - g.y < img.tile_h ? img.height - g.y : img.tile_h);
     return g;
 }
 
-// Window entries are ordered by their first tile.
+// Window entries are ordered by their first tile (scalar loads: every lane
+// of the workgroup asks for the same entries).
 __device__ inline std::uint32_t owner(const tsk_window_entry *e, std::uint32_t n, std::uint32_t tile) {
     std::uint32_t lo = 0, hi = n;
     while (hi - lo > 1) {
@@ -107,6 +123,25 @@ template <bool U16> __device__ inline std::uint32_t residual(std::uint32_t cur, 
     }
 }
 
+typedef std::uint16_t u16x2 __attribute__((ext_vector_type(2)));
+typedef std::int16_t i16x2 __attribute__((ext_vector_type(2)));
+
+// residual<true> of both 16-bit halves at once (packed 16-bit integer ops).
+__device__ inline std::uint32_t residual_pair(std::uint32_t cur, std::uint32_t prev) {
+    const u16x2 d = __builtin_bit_cast(u16x2, cur) - __builtin_bit_cast(u16x2, prev);
+    const u16x2 sign = __builtin_bit_cast(u16x2, __builtin_bit_cast(i16x2, d) >> static_cast<std::int16_t>(15));
+    return __builtin_bit_cast(std::uint32_t, static_cast<u16x2>((d << static_cast<std::uint16_t>(1)) ^ sign));
+}
+
+// ORs `bits` into an LSB-first bit stream at bit `pos`: two LDS ORs, the
+// second of zero when the bits end inside the first word.
+__device__ inline void or_bits(std::uint32_t *stream, std::uint32_t pos, std::uint32_t bits) {
+    const std::uint64_t v = std::uint64_t{bits} << (pos & 31u);
+    std::uint32_t *w = stream + (pos >> 5);
+    atomicOr(w, static_cast<std::uint32_t>(v));
+    atomicOr(w + 1, static_cast<std::uint32_t>(v >> 32));
+}
+
 // floor(p / d) as a multiply-high by ceil(2^32 / d): exact while p * d < 2^32
 // (p below 2^17 and d at most 256 here).
 __device__ inline std::uint32_t magic_of(std::uint32_t d) { return static_cast<std::uint32_t>(0xffffffffu / d) + 1u; }
@@ -126,11 +161,11 @@ __device__ inline std::uint64_t wave_sum(std::uint64_t v) {
     return v;
 }
 
-// Copies the tile into LDS, row r at px + r * stride, one 32-bit word per
+// Copies the tile into LDS at place(r, c), one 16- or 32-bit word per
 // sample.  Every load of a thread is issued before the first one is waited
 // for (a tile is at most 4 chunks of 16 bytes, or 16 samples, per thread).
 template <bool U16>
-__device__ inline void stage_tile(const tsk_encode_image &img, const Geometry &g, std::uint32_t stride,
+__device__ inline void stage_tile(const tsk_encode_image &img, const Geometry &g, std::uint32_t row,
                                   typename Cfg<U16>::sample *px) {
     constexpr std::uint32_t kThreads = Cfg<U16>::threads;
     constexpr std::uint32_t s = U16 ? 2u : 4u;
@@ -158,18 +193,15 @@ __device__ inline void stage_tile(const tsk_encode_image &img, const Geometry &g
             const std::uint32_t k = threadIdx.x + q * kThreads;
             if (k >= chunks) continue;
             const std::uint32_t r = div_magic(k, magic, across), c = k - r * across;
-            typename Cfg<U16>::sample *dst = px + r * stride + c * per;
+            // A chunk never crosses a segment (segments hold a multiple of `per` columns).
+            typename Cfg<U16>::sample *dst = px + place<U16>(r, c * per, row);
             if ((c + 1u) * per <= g.w) {
-                if constexpr (U16) {
-                    // 8 samples at a 4-byte boundary (the stride is even).
-                    std::uint32_t *d32 = reinterpret_cast<std::uint32_t *>(dst);
-                    d32[0] = v[q].x;
-                    d32[1] = v[q].y;
-                    d32[2] = v[q].z;
-                    d32[3] = v[q].w;
-                } else {
-                    reinterpret_cast<u32x4 *>(dst)[0] = v[q];
-                }
+                // 32-bit writes: 8 uint16 samples start at a 4-byte boundary (even places).
+                std::uint32_t *d32 = reinterpret_cast<std::uint32_t *>(dst);
+                d32[0] = v[q].x;
+                d32[1] = v[q].y;
+                d32[2] = v[q].z;
+                d32[3] = v[q].w;
             } else {
                 // The image's right edge: only the samples inside it.
                 const std::uint8_t *p = origin + std::uint64_t{r} * img.in_pitch + c * 16u;
@@ -198,7 +230,7 @@ __device__ inline void stage_tile(const tsk_encode_image &img, const Geometry &g
             for (std::uint32_t q = 0; q < round; ++q) {
                 const std::uint32_t k = k0 + threadIdx.x + q * kThreads;
                 const std::uint32_t r = div_magic(k, magic, g.w), c = k - r * g.w;
-                if (k < n) px[r * stride + c] = static_cast<typename Cfg<U16>::sample>(v[q]);
+                if (k < n) px[place<U16>(r, c, row)] = static_cast<typename Cfg<U16>::sample>(v[q]);
             }
         }
     }
@@ -220,23 +252,18 @@ __global__ void __launch_bounds__(Cfg<U16>::threads) __attribute__((amdgpu_waves
     const std::uint32_t lane = threadIdx.x % 64u;
     const std::uint32_t t = blockIdx.x;
     std::uint64_t *state = a.state + std::uint64_t{a.parity} * a.capacity;
-    {
-        // Clear the other buffer for the next window (the previous one is done with it).
-        std::uint64_t *other = a.state + std::uint64_t{a.parity ^ 1u} * a.capacity;
-        for (std::uint32_t k = t + threadIdx.x * a.tile_count; k < a.capacity; k += kThreads * a.tile_count) other[k] = 0;
-    }
     const tsk_window_entry e = a.entries[owner(a.entries, a.entry_count, t)];
     const tsk_encode_image &img = a.images[e.image];
     const std::uint64_t index = e.start + (t - e.first);
     const Geometry g = geometry(img, index);
-    // Lanes per tile row: threads / tile_h; each lane holds kColumns columns
-    // (tile_w / lanes) of its row.  Row r sits at px + r * stride, with stride
-    // = tile_w + lanes samples so that a wave's reads fall in distinct banks.
+    // Lanes per tile row: threads / tile_h; lane sl of row r owns columns
+    // [sl x kColumns, (sl + 1) x kColumns).
     const std::uint32_t lanes = kThreads / img.tile_h;
-    const std::uint32_t stride = img.tile_w + lanes;
+    const std::uint32_t row = lanes * Cfg<U16>::segment;
     const std::uint32_t r = threadIdx.x / lanes, sl = threadIdx.x % lanes;
+    const std::uint32_t c0 = sl * kColumns;
 
-    stage_tile<U16>(img, g, stride, px);
+    stage_tile<U16>(img, g, row, px);
     __syncthreads();
 
     // ---- analyze -----------------------------------------------------------------
@@ -244,22 +271,59 @@ __global__ void __launch_bounds__(Cfg<U16>::threads) __attribute__((amdgpu_waves
     bool same = true;
     std::uint32_t any = 0, seed = 0;
     // Residuals stay in registers until the packing; uint16 residuals (at
-    // most 16 bits) two to a register.
+    // most 16 bits) two to a register.  The sample before a lane's first
+    // column is the previous lane's last.
     constexpr std::uint32_t kPer = U16 ? 2u : 1u;
     std::uint32_t z[kColumns / kPer];
 #pragma unroll
     for (std::uint32_t i = 0; i < kColumns / kPer; ++i) z[i] = 0;
+    // (Column 0 gets residual 0 against itself.)
+    const bool full = r < g.h && c0 + kColumns <= g.w;
+    if (r < g.h && c0 < g.w) {
+        const Sample *lane_px = px + place<U16>(r, c0, row);
+        const std::uint32_t before = lane_px[c0 != 0 ? -static_cast<int>(Cfg<U16>::segment - kColumns + 1u) : 0];
+        seed = lane_px[0];
+        if (full) {
+            // Every column valid: no per-column conditions.
+            std::uint32_t diff = 0;
+            if constexpr (U16) {
+                const std::uint32_t *lane_w = reinterpret_cast<const std::uint32_t *>(lane_px); // even place
+                const std::uint32_t first2 = first * 0x10001u;
+                std::uint32_t prev_w = before << 16;
 #pragma unroll
-    for (std::uint32_t i = 0; i < kColumns; ++i) {
-        const std::uint32_t c = sl + lanes * i;
-        if (r < g.h && c < g.w) {
-            const std::uint32_t v = px[r * stride + c];
-            same = same && v == first;
-            if (c == 0) seed = v;
-            if (c != 0) {
-                const std::uint32_t zi = residual<U16>(v, px[r * stride + c - 1u]);
-                z[i / kPer] |= zi << (U16 ? 16u * (i % 2u) : 0u);
-                any |= zi;
+                for (std::uint32_t j = 0; j < kColumns / 2u; ++j) {
+                    const std::uint32_t w = lane_w[j];
+                    // Previous samples of the two halves: prev_w's high half and w's low half.
+                    z[j] = residual_pair(w, __builtin_amdgcn_alignbit(w, prev_w, 16));
+                    any |= z[j];
+                    diff |= w ^ first2;
+                    prev_w = w;
+                }
+                any = (any | (any >> 16)) & 0xffffu;
+            } else {
+                std::uint32_t prev = before;
+#pragma unroll
+                for (std::uint32_t i = 0; i < kColumns; ++i) {
+                    const std::uint32_t v = lane_px[i];
+                    z[i] = v ^ prev;
+                    any |= z[i];
+                    diff |= v ^ first;
+                    prev = v;
+                }
+            }
+            same = diff == 0;
+        } else {
+            std::uint32_t prev = before;
+#pragma unroll
+            for (std::uint32_t i = 0; i < kColumns; ++i) {
+                if (c0 + i < g.w) {
+                    const std::uint32_t v = lane_px[i];
+                    same = same && v == first;
+                    const std::uint32_t zi = residual<U16>(v, prev);
+                    z[i / kPer] |= zi << (U16 ? 16u * (i % 2u) : 0u);
+                    any |= zi;
+                    prev = v;
+                }
             }
         }
     }
@@ -314,21 +378,40 @@ __global__ void __launch_bounds__(Cfg<U16>::threads) __attribute__((amdgpu_waves
         __syncthreads();
         if (r < g.h) {
             // Each lane ORs its residuals (still in registers) into the row's
-            // LSB-first bit stream: residual of column c at bit (c - 1) * b.
-            const std::uint32_t b = row_bits[r], base = row_word[r];
+            // LSB-first bit stream, residual of column c at bit (c - 1) x b.
+            // Every write to the zeroed payload is an OR, so lanes sharing a
+            // word (or an OR of zero past a row's end) need no ordering.
+            const std::uint32_t b = row_bits[r], base = row_word[r] + 2u;
             if (sl == 0) {
-                lds[base] = seed;
-                lds[base + 1u] = b;
+                atomicOr(&lds[base - 2u], seed);
+                atomicOr(&lds[base - 1u], b);
             }
+            std::uint32_t *stream = lds + base;
+            if (b != 0 && full) {
+                if constexpr (U16) {
+                    // Residual pairs (2 b <= 32 bits) of columns c0 + 2j, c0 + 2j + 1.
 #pragma unroll
-            for (std::uint32_t i = 0; i < kColumns; ++i) {
-                const std::uint32_t c = sl + lanes * i;
-                const std::uint32_t zi = U16 ? (z[i / kPer] >> (16u * (i % 2u))) & 0xffffu : z[i];
-                if (zi != 0 && c < g.w) {
-                    const std::uint32_t pos = (c - 1u) * b;
-                    const std::uint32_t w = base + 2u + (pos >> 5), sh = pos & 31u;
-                    atomicOr(&lds[w], zi << sh);
-                    if (sh + b > 32u) atomicOr(&lds[w + 1u], zi >> (32u - sh));
+                    for (std::uint32_t j = 0; j < kColumns / 2u; ++j) {
+                        const std::uint32_t hi = z[j] >> 16;
+                        std::uint32_t bits = (z[j] & 0xffffu) | (hi << b);
+                        std::uint32_t pos = (c0 + 2u * j - 1u) * b;
+                        if (j == 0 && c0 == 0) {
+                            bits = hi; // column 0 carries no residual
+                            pos = 0;
+                        }
+                        or_bits(stream, pos, bits);
+                    }
+                } else {
+#pragma unroll
+                    for (std::uint32_t i = 0; i < kColumns; ++i)
+                        if (i != 0 || c0 != 0) or_bits(stream, (c0 + i - 1u) * b, z[i]);
+                }
+            } else if (b != 0 && c0 < g.w) {
+#pragma unroll
+                for (std::uint32_t i = 0; i < kColumns; ++i) {
+                    const std::uint32_t c = c0 + i;
+                    if (c != 0 && c < g.w)
+                        or_bits(stream, (c - 1u) * b, U16 ? (z[i / kPer] >> (16u * (i % 2u))) & 0xffffu : z[i]);
                 }
             }
         }
@@ -402,14 +485,14 @@ __global__ void __launch_bounds__(Cfg<U16>::threads) __attribute__((amdgpu_waves
             std::uint32_t v;
             if constexpr (U16) {
                 const std::uint32_t p0 = 2u * k, r0 = div_magic(p0, magic, g.w);
-                v = px[r0 * stride + (p0 - r0 * g.w)];
+                v = px[place<U16>(r0, p0 - r0 * g.w, row)];
                 if (p0 + 1u < n) {
                     const std::uint32_t r1 = div_magic(p0 + 1u, magic, g.w);
-                    v |= std::uint32_t{px[r1 * stride + (p0 + 1u - r1 * g.w)]} << 16;
+                    v |= std::uint32_t{px[place<U16>(r1, p0 + 1u - r1 * g.w, row)]} << 16;
                 }
             } else {
                 const std::uint32_t r0 = div_magic(k, magic, g.w);
-                v = px[r0 * stride + (k - r0 * g.w)];
+                v = px[place<U16>(r0, k - r0 * g.w, row)];
             }
             dst[k] = v;
         }
@@ -451,6 +534,12 @@ __global__ void __launch_bounds__(Cfg<U16>::threads) __attribute__((amdgpu_waves
         }
     }
 
+    {
+        // Clear the other buffer for the next window (the previous one is done
+        // with it), last: a store in flight would hold up this tile's first loads.
+        std::uint64_t *other = a.state + std::uint64_t{a.parity ^ 1u} * a.capacity;
+        for (std::uint32_t k = t + threadIdx.x * a.tile_count; k < a.capacity; k += kThreads * a.tile_count) other[k] = 0;
+    }
 }
 
 __global__ void clear_kernel(std::uint64_t *state, std::uint64_t words) {
