@@ -138,6 +138,9 @@ typedef struct {
 typedef struct {
     const tsk_encode_image *images;
     const tsk_window_entry *entries;
+    /* Per block of 64 window tiles: the entry holding the block's first tile
+     * (count ceil(tile_count / 64)); NULL: search the entries. */
+    const uint32_t *blocks;
     void *results;               /* DeviceResult per image */
     uint64_t *state;             /* workspace: 2 x capacity look-back words, one buffer per window parity */
     uint64_t *carry;             /* workspace: 2 slots, output cursor of a continued image */

@@ -252,7 +252,15 @@ __global__ void __launch_bounds__(Cfg<U16>::threads) __attribute__((amdgpu_waves
     const std::uint32_t lane = threadIdx.x % 64u;
     const std::uint32_t t = blockIdx.x;
     std::uint64_t *state = a.state + std::uint64_t{a.parity} * a.capacity;
-    const tsk_window_entry e = a.entries[owner(a.entries, a.entry_count, t)];
+    std::uint32_t o;
+    if (a.blocks != nullptr) {
+        // The block's entry, then forward over entries starting later in the block.
+        o = a.blocks[t >> 6];
+        while (o + 1u < a.entry_count && a.entries[o + 1u].first <= t) ++o;
+    } else {
+        o = owner(a.entries, a.entry_count, t);
+    }
+    const tsk_window_entry e = a.entries[o];
     const tsk_encode_image &img = a.images[e.image];
     const std::uint64_t index = e.start + (t - e.first);
     const Geometry g = geometry(img, index);

@@ -505,6 +505,17 @@ Status build(const BatchRequest &request, PlanState &plan, hipStream_t stream) {
             first += d.tiles[image];
         }
     }
+    // Tile-to-entry tables: one per window, then the whole-batch window's.
+    std::vector<std::uint32_t> blocks;
+    std::vector<std::size_t> blocks_first;
+    for (const auto &[first, w] : windows.windows) {
+        blocks_first.push_back(blocks.size());
+        append_block_owners(windows.entries.data() + first, w.entry_count, w.tile_count, blocks);
+    }
+    const std::size_t whole_blocks_at = blocks.size();
+    if (whole)
+        append_block_owners(windows.entries.data() + whole_at, d.tiles.size(), static_cast<std::uint32_t>(total_tiles),
+                            blocks);
     plan.spec = {0, 0, d.shape, 0};
     for (const auto &[first, w] : windows.windows) {
         plan.spec.image_count = std::max<std::uint64_t>(plan.spec.image_count, w.entry_count);
@@ -517,12 +528,14 @@ Status build(const BatchRequest &request, PlanState &plan, hipStream_t stream) {
         std::uint64_t bytes = 0;
         return host::round_up(total, 16, at) && host::mul(count, size, bytes) && host::add(at, bytes, total);
     };
-    std::uint64_t dec_images_at = 0, tasks_at = 0, rows_at = 0, enc_images_at = 0, entries_at = 0, cal_at = 0, first_at = 0;
+    std::uint64_t dec_images_at = 0, tasks_at = 0, rows_at = 0, enc_images_at = 0, entries_at = 0, cal_at = 0, first_at = 0,
+                  blocks_at = 0;
     if (!place(d.decode.images.size(), sizeof(tsk_decode_image), dec_images_at) ||
         !place(d.decode.tasks.size(), sizeof(tsk_decode_task), tasks_at) ||
         !place(d.decode.rows.size(), sizeof(std::uint16_t), rows_at) ||
         !place(d.encode.size(), sizeof(tsk_encode_image), enc_images_at) ||
         !place(windows.entries.size(), sizeof(tsk_window_entry), entries_at) ||
+        !place(blocks.size(), sizeof(std::uint32_t), blocks_at) ||
         !place(d.calibrate.size(), sizeof(tsk_calibrate_image), cal_at) ||
         !place(d.first_rows.size(), sizeof(std::uint64_t), first_at))
         return Status::size_overflow;
@@ -539,13 +552,15 @@ Status build(const BatchRequest &request, PlanState &plan, hipStream_t stream) {
     plan.calibrate_images = reinterpret_cast<const tsk_calibrate_image *>(base + cal_at);
     plan.first_rows = reinterpret_cast<const std::uint64_t *>(base + first_at);
     plan.total_rows = d.total_rows;
-    for (const auto &[first, w] : windows.windows) {
-        EncodeWindow placed = w;
-        placed.entries = reinterpret_cast<const tsk_window_entry *>(base + entries_at) + first;
+    for (std::size_t k = 0; k < windows.windows.size(); ++k) {
+        EncodeWindow placed = windows.windows[k].second;
+        placed.entries = reinterpret_cast<const tsk_window_entry *>(base + entries_at) + windows.windows[k].first;
+        placed.blocks = reinterpret_cast<const std::uint32_t *>(base + blocks_at) + blocks_first[k];
         plan.windows.push_back(placed);
     }
     if (whole) {
         plan.whole.entries = reinterpret_cast<const tsk_window_entry *>(base + entries_at) + whole_at;
+        plan.whole.blocks = reinterpret_cast<const std::uint32_t *>(base + blocks_at) + whole_blocks_at;
         plan.whole.entry_count = static_cast<std::uint32_t>(d.tiles.size());
         plan.whole.tile_count = static_cast<std::uint32_t>(total_tiles);
     }
@@ -558,6 +573,7 @@ Status build(const BatchRequest &request, PlanState &plan, hipStream_t stream) {
         !upload(plan.storage, tasks_at, d.decode.tasks, stream) || !upload(plan.storage, rows_at, d.decode.rows, stream) ||
         !upload(plan.storage, enc_images_at, d.encode, stream) ||
         !upload(plan.storage, entries_at, windows.entries, stream) ||
+        !upload(plan.storage, blocks_at, blocks, stream) ||
         !upload(plan.storage, cal_at, d.calibrate, stream) || !upload(plan.storage, first_at, d.first_rows, stream) ||
         hipStreamSynchronize(stream) != hipSuccess)
         return hip_failure();
@@ -689,6 +705,7 @@ Status compress_async(const BatchPlan &plan, MutableBytes workspace, ResultBuffe
     if (state->whole.tile_count != 0) {
         a.capacity = state->whole.tile_count;
         a.entries = state->whole.entries;
+        a.blocks = state->whole.blocks;
         a.entry_count = state->whole.entry_count;
         a.tile_count = state->whole.tile_count;
         a.parity = 0;
@@ -701,6 +718,7 @@ Status compress_async(const BatchPlan &plan, MutableBytes workspace, ResultBuffe
     for (std::size_t k = 0; k < state->windows.size(); ++k) {
         const detail::EncodeWindow &w = state->windows[k];
         a.entries = w.entries;
+        a.blocks = w.blocks;
         a.entry_count = w.entry_count;
         a.tile_count = w.tile_count;
         a.parity = static_cast<std::uint32_t>(k & 1u);

@@ -35,9 +35,22 @@ class RangeSet {
 
 struct EncodeWindow {
     const tsk_window_entry *entries = nullptr; // device
+    const std::uint32_t *blocks = nullptr;     // device: entry of window tile 64 b
     std::uint32_t entry_count = 0;
     std::uint32_t tile_count = 0;
 };
+
+// Appends, for each block of 64 tiles of a window, the index of the entry
+// (among the window's `count` entries, ordered by first tile) holding the
+// block's first tile: the encoder's lookup from tile to entry.
+inline void append_block_owners(const tsk_window_entry *entries, std::size_t count, std::uint32_t tiles,
+                                std::vector<std::uint32_t> &out) {
+    std::size_t e = 0;
+    for (std::uint64_t tile = 0; tile < tiles; tile += 64) {
+        while (e + 1 < count && entries[e + 1].first <= tile) ++e;
+        out.push_back(static_cast<std::uint32_t>(e));
+    }
+}
 
 // Launch shape of the decode kernel.
 struct DecodeShape {

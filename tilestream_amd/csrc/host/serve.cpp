@@ -186,7 +186,7 @@ namespace {
 // (region requests first, then encode requests) | tsk_decode_image[regions] |
 // tsk_request[regions] | request of every decode task | staging block of every
 // decode task | tsk_encode_image[encodes] (uint16 images, then float32) |
-// encode window entries.
+// encode window entries | per window, the entry of each block of 64 tiles.
 struct Slot {
     hipStream_t stream = nullptr;
     hipEvent_t done = nullptr;
@@ -204,10 +204,11 @@ struct Slot {
 constexpr std::uint64_t kCounterBytes = 256;
 
 struct Layout {
-    std::uint64_t results, images, requests, index, staged, encode, entries, total;
+    std::uint64_t results, images, requests, index, staged, encode, entries, blocks, total;
 };
 
-Layout layout_for(std::uint64_t regions, std::uint64_t encodes, std::uint64_t tasks, std::uint64_t entries) {
+Layout layout_for(std::uint64_t regions, std::uint64_t encodes, std::uint64_t tasks, std::uint64_t entries,
+                  std::uint64_t blocks) {
     Layout l{};
     l.results = kCounterBytes;
     l.images = align_up(l.results + (regions + encodes) * sizeof(DeviceResult), 256);
@@ -216,7 +217,8 @@ Layout layout_for(std::uint64_t regions, std::uint64_t encodes, std::uint64_t ta
     l.staged = align_up(l.index + tasks * sizeof(std::uint32_t), 256);
     l.encode = align_up(l.staged + tasks * sizeof(std::uint32_t), 256);
     l.entries = align_up(l.encode + encodes * sizeof(tsk_encode_image), 256);
-    l.total = align_up(l.entries + entries * sizeof(tsk_window_entry), 16);
+    l.blocks = align_up(l.entries + entries * sizeof(tsk_window_entry), 16);
+    l.total = align_up(l.blocks + blocks * sizeof(std::uint32_t), 16);
     return l;
 }
 
@@ -286,7 +288,8 @@ struct Server::Impl {
 
     [[nodiscard]] Layout capacity() const noexcept {
         const std::uint64_t n = options.max_batch;
-        return layout_for(n, n, options.max_tasks, n + 2 * (options.max_tasks / options.encode_window + 2));
+        const std::uint64_t windows = 2 * (options.max_tasks / options.encode_window + 2);
+        return layout_for(n, n, options.max_tasks, n + windows, options.max_tasks / 64 + windows);
     }
     Status extent(const ServeRequest &r, Extent &e) const noexcept;
     // Waits for every batch in flight and frees the slots.
@@ -443,7 +446,18 @@ Status Server::Impl::launch(Slot &slot, const ServeRequest *requests, std::uint6
     split_windows(enc_tiles[0], options.encode_window, entries, windows[0]);
     const std::size_t group1_entries = entries.size();
     split_windows(enc_tiles[1], options.encode_window, entries, windows[1]);
-    const Layout l = layout_for(regions, encodes[0] + encodes[1], tasks, entries.size());
+    std::vector<std::uint32_t> blocks;
+    std::vector<std::size_t> blocks_first[2];
+    for (int g = 0; g < 2; ++g) {
+        const std::size_t end_entries = g == 0 ? group1_entries : entries.size();
+        for (std::size_t w = 0; w < windows[g].size(); ++w) {
+            const std::size_t e0 = windows[g][w].first;
+            const std::size_t e1 = w + 1 < windows[g].size() ? windows[g][w + 1].first : end_entries;
+            blocks_first[g].push_back(blocks.size());
+            detail::append_block_owners(entries.data() + e0, e1 - e0, windows[g][w].second, blocks);
+        }
+    }
+    const Layout l = layout_for(regions, encodes[0] + encodes[1], tasks, entries.size(), blocks.size());
 
     std::uint8_t *h = slot.host;
     std::memset(h, 0, l.images); // work counter and results start at zero
@@ -457,6 +471,7 @@ Status Server::Impl::launch(Slot &slot, const ServeRequest *requests, std::uint6
     auto *enc = reinterpret_cast<tsk_encode_image *>(h + l.encode);
     if (!entries.empty())
         std::memcpy(h + l.entries, entries.data(), entries.size() * sizeof(tsk_window_entry));
+    if (!blocks.empty()) std::memcpy(h + l.blocks, blocks.data(), blocks.size() * sizeof(std::uint32_t));
     slot.order.assign(regions + encodes[0] + encodes[1], 0);
     bool u16 = false, f32 = false, cal = false, plain = false;
     std::uint64_t task = 0, j = 0, e_next[2] = {0, encodes[0]};
@@ -592,6 +607,7 @@ Status Server::Impl::launch(Slot &slot, const ServeRequest *requests, std::uint6
                 const std::size_t e0 = windows[g][w].first;
                 const std::size_t e1 = w + 1 < windows[g].size() ? windows[g][w + 1].first : end_entries;
                 a.entries = reinterpret_cast<const tsk_window_entry *>(slot.device + l.entries) + e0;
+                a.blocks = reinterpret_cast<const std::uint32_t *>(slot.device + l.blocks) + blocks_first[g][w];
                 a.entry_count = static_cast<std::uint32_t>(e1 - e0);
                 a.tile_count = windows[g][w].second;
                 a.parity = sequence++ & 1u;
