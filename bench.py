@@ -252,12 +252,14 @@ def roi_leg(args, threads: int, stream):
         assert api.decode_calibrate_async(plan, ws, res, 1, stream) == api.Status.success
         stream.synchronize()
         plain_lat.append(time.perf_counter() - t1)
-    plain = {"requests": sample, "p50_us": round(percentile(plain_lat, 50) * 1e6, 1),
-             "p99_us": round(percentile(plain_lat, 99) * 1e6, 1),
-             "requests_per_s": round(sample / sum(plain_lat), 1),
-             "note": "one prepare_batch + decode_calibrate_async + stream sync per request, host planning included"}
-    log(f"roi per-request path: p50 {plain['p50_us']} us, p99 {plain['p99_us']} us, "
-        f"{plain['requests_per_s']} requests/s")
+    plain = None
+    if sample > 0:
+        plain = {"requests": sample, "p50_us": round(percentile(plain_lat, 50) * 1e6, 1),
+                 "p99_us": round(percentile(plain_lat, 99) * 1e6, 1),
+                 "requests_per_s": round(sample / sum(plain_lat), 1),
+                 "note": "one prepare_batch + decode_calibrate_async + stream sync per request, host planning included"}
+        log(f"roi per-request path: p50 {plain['p50_us']} us, p99 {plain['p99_us']} us, "
+            f"{plain['requests_per_s']} requests/s")
 
     verified = None
     if not args.no_verify:
@@ -908,10 +910,51 @@ def fused_leg(args, stream):
         }
         log(f"fused {kind}: {ms:.4f} ms, {out[kind]['algorithmic_GBps']} GB/s algorithmic, "
             f"max {out[kind]['fp64_error']['max_ulp']} ulp")
-        del keep, plans
+        # Standalone calibration of the raw image with the same maps (reference
+        # calibrate_async): the same rotating sets; its output must equal the
+        # fused decode's word for word.
+        cal_plans, cal_keep = [], []
+        cal_ws = 256
+        ptype = api.PixelType.uint16 if kind == "u16" else api.PixelType.float32
+        for k in range(sets):
+            _, om, gm, _, _ = keep[k]
+            raw = api.DeviceMemory.from_host(px)
+            dst = api.DeviceMemory(side * side * 4)
+            it = api.BatchItem()
+            it.input = api.device_view(raw, ptype, side, side, const=True)
+            it.output = api.device_view(dst, api.PixelType.float32, side, side)
+            it.calibration.offset = api.map_parameter(api.device_view(om, api.PixelType.float32, side, side, const=True))
+            it.calibration.gain = api.map_parameter(api.device_view(gm, api.PixelType.float32, side, side, const=True))
+            st, plan = api.prepare_batch(api.Operation.calibrate, [it], stream=stream)
+            assert st == api.Status.success, st
+            cal_ws = max(cal_ws, plan.workspace_bytes())
+            cal_plans.append(plan)
+            cal_keep.append((raw, dst, it))
+        ws2 = api.DeviceMemory(cal_ws)
+        step2 = [0]
+
+        def run_cal():
+            api.calibrate_async(cal_plans[step2[0] % sets], ws2, res, 1, stream)
+            step2[0] += 1
+
+        for _ in range(args.warmup):
+            run_cal()
+        cal_ms = timed(stream, run_cal, args.steps)
+        stream.synchronize()
+        cal_status, _ = api.read_results(res, 1)
+        cal_got = cal_keep[(step2[0] - 1) % sets][1].download().view(np.uint32).reshape(side, side)
+        cal_alg = side * side * (px.itemsize + 4 + 8)
+        out[kind]["standalone_calibrate"] = {
+            "ms": round(cal_ms, 4), "algorithmic_GBps": round(cal_alg / (cal_ms * 1e-3) / 1e9, 1),
+            "roofline_frac": round(cal_alg / (cal_ms * 1e-3) / 1e9 / HBM_PEAK_GBPS, 4), "algorithmic_bytes": cal_alg,
+            "status": int(cal_status[0]), "equals_fused_bitwise": bool(np.array_equal(cal_got, got)),
+            "note": "calibrate_async of the raw image with the same maps: pixels + FP32 output + two FP32 maps"}
+        log(f"calibrate {kind}: {cal_ms:.4f} ms, {out[kind]['standalone_calibrate']['algorithmic_GBps']} GB/s "
+            f"algorithmic, equals fused {out[kind]['standalone_calibrate']['equals_fused_bitwise']}")
+        del keep, plans, cal_keep, cal_plans
     return {"workload": "4096x4096 uint16 detector frame and 4096x4096 float32 field, 64x64 tiles, whole "
                         "decode fused with dark map U[90,110] and gain map N(1,0.02), one plan per image, "
-                        "4 rotating buffer sets", **out}
+                        "4 rotating buffer sets; beside it, standalone calibration of the raw images", **out}
 
 
 def main() -> int:
@@ -934,6 +977,7 @@ def main() -> int:
                    help="output rows start at multiples of this many bytes, as hipMallocPitch "
                         "allocates them (1: packed rows)")
     p.add_argument("--no-roi", action="store_true", help="skip the ROI serving leg")
+    p.add_argument("--fused-only", action="store_true", help="only the fused and standalone calibration leg")
     p.add_argument("--no-fused", action="store_true", help="skip the fused-calibration real-shape leg")
     p.add_argument("--roi-only", action="store_true", help="only the ROI serving leg")
     p.add_argument("--roi-archives", type=int, default=64)
@@ -1001,6 +1045,11 @@ def main() -> int:
         mix = serve_mix_leg(args, threads, api.Stream())
         print(json.dumps({"metric": "serving mix throughput and latency", "serve_mix": mix}), flush=True)
         return 0 if mix["verified_bit_exact"] in (None, True) else 3
+    if args.fused_only:
+        fu = fused_leg(args, api.Stream())
+        ok = all(v["standalone_calibrate"]["equals_fused_bitwise"] for k, v in fu.items() if k in ("u16", "f32"))
+        print(json.dumps({"metric": "fused decode and calibration", "fused_real_shape": fu}), flush=True)
+        return 0 if ok else 3
     if args.roi_only:
         roi = roi_leg(args, threads, api.Stream())
         print(json.dumps({"metric": "p99 ROI latency", "roi": roi}), flush=True)
