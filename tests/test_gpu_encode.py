@@ -92,7 +92,8 @@ def test_every_width(orc):
 
 
 def test_image_split_across_windows(orc):
-    """14,005 tiles: the output cursor carries across 4096-tile windows."""
+    """14,005 tiles of one image in one plan (the plan's whole-batch window; windows that carry
+    the output cursor are covered by the serving tests with 64- and 200-tile windows)."""
     px = g.detector_u16(128 * 5, 32 * 2801, 62)
     data = Encoded([px], (128, 32)).run()[0]
     assert data == orc.compress(px, (128, 32))
