@@ -468,8 +468,9 @@ def serve_mix_leg(args, threads: int, stream):
         f"set up in {time.perf_counter() - t0:.1f} s")
 
     shape = tuple(int(v) for v in args.roi_shape.split(","))
-    server = api.Server(aset, api.server_options(streams=args.mix_streams, max_batch=args.roi_max_batch,
-                                                 admission_bytes=200 * 10**9, shape=shape))
+    opts = api.server_options(streams=args.mix_streams, max_batch=args.roi_max_batch,
+                              admission_bytes=200 * 10**9, shape=shape)
+    server = api.Server(aset, opts)
     sat = None
     for _ in range(2):
         st, lat, statuses, written, stats = server.replay(requests, np.zeros(n, np.uint64))
@@ -537,7 +538,10 @@ def serve_mix_leg(args, threads: int, stream):
                        "bytes": int(sat.decoded_bytes),
                        "note": "bytes = pixels decoded (stored sample size) + pixels encoded"},
         "latency": latency,
-        "server": {"streams": args.mix_streams, "max_batch": args.roi_max_batch, "shape": list(shape)},
+        "server": {"streams": args.mix_streams, "max_batch": args.roi_max_batch, "shape": list(shape),
+                   "large_bytes": int(opts.large_bytes),
+                   "note": "requests with at least large_bytes of output batch apart, on streams - 1 at most, "
+                           "with (streams - 1) / streams of the decode grid"},
         "verified_bit_exact": verified,
     }
 

@@ -89,6 +89,10 @@ struct ServerOptions {
     std::uint64_t admission_bytes = 200ull << 30; // output bytes in flight (encode: its compress bound)
     std::uint64_t staging_bytes = 256ull << 20;   // per stream: device staging for tiles of host archives
     LaunchShape shape;                   // decode kernel launch shape (grid 0: sized per batch)
+    // A request whose output bytes (encode: its compress bound) reach this is
+    // large: it is batched only with large requests, and large batches hold at
+    // most streams - 1 streams, so small requests never wait behind a large one.
+    std::uint64_t large_bytes = 32ull << 20;
 };
 
 struct ServerStats {
@@ -112,7 +116,8 @@ class Server {
     [[nodiscard]] Status check(const ServeRequest &request) const noexcept;
 
     // Open-loop replay: request i arrives arrival_ns[i] nanoseconds after the
-    // start (non-decreasing).  latency_ns[i] (optional) receives completion -
+    // start (non-decreasing); requests of one size class (large_bytes) are
+    // admitted in arrival order, the older head of the two classes first.  latency_ns[i] (optional) receives completion -
     // arrival as seen by the host, results[i] (optional, host memory) the
     // request's device result (status; bytes_written: output bytes, the file
     // size of an encode).  Every request is validated before the replay starts;

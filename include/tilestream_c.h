@@ -175,6 +175,7 @@ typedef struct {
     uint64_t admission_bytes;
     uint64_t staging_bytes;        /* per stream, device staging for tiles of host archives */
     ts_launch_shape shape;
+    uint64_t large_bytes;          /* requests with this many output bytes batch apart, on streams - 1 at most */
 } ts_server_options;
 typedef struct { uint64_t requests, batches, decoded_bytes, failed; double seconds; } ts_server_stats;
 uint32_t ts_archive_set_create(ts_archive_set **set);

@@ -142,13 +142,16 @@ def test_alone_and_in_the_mix_are_identical(resident):
         assert np.array_equal(alone.read([specs[i]], 0), mix.read(specs, i)), (i, specs[i])
 
 
-def test_replay_with_poisson_arrivals(orc, resident):
+@pytest.mark.parametrize("large", [None, 64 << 10])
+def test_replay_with_poisson_arrivals(orc, resident, large):
+    """At the default size-class bound every region is small; at 64 KiB the
+    larger ones batch apart on at most two of the three streams."""
     rng = np.random.default_rng(12)
     specs = _requests(resident, rng, 200)
     out = Outputs(resident, specs)
     arrivals = np.cumsum(rng.exponential(20_000.0, len(specs))).astype(np.uint64)  # 50k requests/s
     server = api.Server(resident.set, api.server_options(streams=3, max_batch=32,
-                                                         admission_bytes=4 << 20))
+                                                         admission_bytes=4 << 20, large_bytes=large))
     st, latency, statuses, _, stats = server.replay(out.requests, arrivals)
     assert st == ts.Status.success
     assert (statuses == 0).all() and stats.failed == 0
@@ -282,7 +285,8 @@ def test_encode_requests_match_oracle_in_a_mixed_batch(orc, resident):
         assert written[regions[i]] == spec[1][2] * spec[1][3] * (4 if spec[2] else resident.pixels[spec[0]].itemsize)
 
 
-def test_encode_alone_and_in_the_mix_are_identical(resident):
+@pytest.mark.parametrize("large", [None, 256 << 10])
+def test_encode_alone_and_in_the_mix_are_identical(resident, large):
     rng = np.random.default_rng(405)
     specs = _requests(resident, rng, 24)
     out = Outputs(resident, specs)
@@ -291,7 +295,8 @@ def test_encode_alone_and_in_the_mix_are_identical(resident):
     order = np.random.default_rng(406).permutation(len(requests))
     mixed = [requests[i] for i in order]
     arrivals = np.cumsum(np.random.default_rng(407).exponential(30_000.0, len(mixed))).astype(np.uint64)
-    server = api.Server(resident.set, api.server_options(streams=3, max_batch=6, admission_bytes=8 << 20))
+    server = api.Server(resident.set, api.server_options(streams=3, max_batch=6, admission_bytes=8 << 20,
+                                                         large_bytes=large))
     st, latency, statuses, written, stats = server.replay(mixed, arrivals)
     assert st == ts.Status.success and (statuses == 0).all() and stats.failed == 0
     pos = {int(i): k for k, i in enumerate(order)}

@@ -199,6 +199,7 @@ struct Slot {
     std::vector<std::uint64_t> order;  // request index of each result of the current batch
     std::uint64_t bytes = 0;           // output bytes admitted
     bool busy = false;
+    bool large = false;                // holds a batch of large requests
 };
 
 constexpr std::uint64_t kCounterBytes = 256;
@@ -299,7 +300,7 @@ struct Server::Impl {
         for (Slot &slot : slots) slot.busy = false;
     }
     // Allocates its window tables: may throw std::bad_alloc (replay turns it into resource_limit).
-    Status launch(Slot &slot, const ServeRequest *requests, std::uint64_t first, std::uint64_t count);
+    Status launch(Slot &slot, const ServeRequest *requests, const std::uint64_t *ids, std::uint64_t count, bool large);
 };
 
 Server::Server(std::unique_ptr<Impl> impl) noexcept : impl_(std::move(impl)) {}
@@ -426,19 +427,20 @@ Status Server::check(const ServeRequest &request) const noexcept {
 // Stages the batch's tables, copies them to the device with one kernel, runs
 // the region decode in request mode and the encode windows, and copies the
 // final results into pinned memory; then marks the end.
-Status Server::Impl::launch(Slot &slot, const ServeRequest *requests, std::uint64_t first, std::uint64_t count) {
+Status Server::Impl::launch(Slot &slot, const ServeRequest *requests, const std::uint64_t *ids, std::uint64_t count,
+                            bool large) {
     std::uint64_t regions = 0, tasks = 0;
     std::uint64_t encodes[2] = {0, 0}; // uint16, float32
     std::vector<std::uint64_t> enc_tiles[2];
     for (std::uint64_t k = 0; k < count; ++k) {
-        const ServeRequest &r = requests[first + k];
+        const ServeRequest &r = requests[ids[k]];
         if (r.kind == RequestKind::region) {
             ++regions;
-            tasks += extents[first + k].tasks;
+            tasks += extents[ids[k]].tasks;
         } else {
             const int g = r.input.image.type == PixelType::uint16 ? 0 : 1;
             ++encodes[g];
-            enc_tiles[g].push_back(extents[first + k].tasks);
+            enc_tiles[g].push_back(extents[ids[k]].tasks);
         }
     }
     std::vector<tsk_window_entry> entries;
@@ -463,7 +465,7 @@ Status Server::Impl::launch(Slot &slot, const ServeRequest *requests, std::uint6
     std::memset(h, 0, l.images); // work counter and results start at zero
     auto *staged = reinterpret_cast<std::uint32_t *>(h + l.staged);
     bool staging = false;
-    for (std::uint64_t k = 0; k < count && !staging; ++k) staging = extents[first + k].staged != 0;
+    for (std::uint64_t k = 0; k < count && !staging; ++k) staging = extents[ids[k]].staged != 0;
     std::uint64_t staged_block = 0;
     auto *images = reinterpret_cast<tsk_decode_image *>(h + l.images);
     auto *reqs = reinterpret_cast<tsk_request *>(h + l.requests);
@@ -476,8 +478,8 @@ Status Server::Impl::launch(Slot &slot, const ServeRequest *requests, std::uint6
     bool u16 = false, f32 = false, cal = false, plain = false;
     std::uint64_t task = 0, j = 0, e_next[2] = {0, encodes[0]};
     for (std::uint64_t k = 0; k < count; ++k) {
-        const ServeRequest &r = requests[first + k];
-        const Extent &e = extents[first + k];
+        const ServeRequest &r = requests[ids[k]];
+        const Extent &e = extents[ids[k]];
         if (r.kind == RequestKind::encode) {
             const int g = r.input.image.type == PixelType::uint16 ? 0 : 1;
             const std::uint64_t at = e_next[g]++;
@@ -493,7 +495,7 @@ Status Server::Impl::launch(Slot &slot, const ServeRequest *requests, std::uint6
             im.tile_w = r.tile.width;
             im.tile_h = r.tile.height;
             im.type = static_cast<std::uint32_t>(r.input.image.type);
-            slot.order[regions + at] = first + k;
+            slot.order[regions + at] = ids[k];
             continue;
         }
         const Archive &a = set->archives[r.archive];
@@ -538,7 +540,7 @@ Status Server::Impl::launch(Slot &slot, const ServeRequest *requests, std::uint6
             }
         }
         task += e.tasks;
-        slot.order[j] = first + k;
+        slot.order[j] = ids[k];
         ++j;
         u16 = u16 || a.info.image.type == PixelType::uint16;
         f32 = f32 || a.info.image.type == PixelType::float32;
@@ -555,8 +557,12 @@ Status Server::Impl::launch(Slot &slot, const ServeRequest *requests, std::uint6
         if (u16 != f32 && cal != plain)
             kind = u16 ? (cal ? TSK_KIND_U16_CAL : TSK_KIND_U16) : (cal ? TSK_KIND_F32_CAL : TSK_KIND_F32);
         const LaunchShape &sh = options.shape;
-        const std::uint64_t resident =
+        std::uint64_t resident =
             static_cast<std::uint64_t>(cus) * static_cast<std::uint64_t>(std::max(1, per_cu[kind]));
+        // A batch of large requests is a persistent grid that holds its CUs to
+        // the end: it takes (streams - 1) / streams of them, so batches of small
+        // requests on the other streams find CUs free.
+        if (large && slots.size() > 1) resident = std::max<std::uint64_t>(1, resident * (slots.size() - 1) / slots.size());
         const std::uint64_t grid = sh.grid != 0 ? sh.grid : std::min<std::uint64_t>(resident, tasks);
         const bool queued = sh.chunk != 0 && tasks > grid;
         tsk_decode_args args{};
@@ -674,12 +680,17 @@ Status Server::replay(const ServeRequest *requests, const std::uint64_t *arrival
         auto elapsed = [&] {
             return static_cast<std::uint64_t>(std::chrono::duration_cast<std::chrono::nanoseconds>(clock::now() - start).count());
         };
+        // Arrived requests wait in two queues by size class, in arrival order.
+        std::vector<std::uint64_t> queue[2]; // 0: small, 1: large (output bytes >= large_bytes)
+        std::size_t head[2] = {0, 0};
         std::uint64_t arrived = 0, dispatched = 0, finished = 0, in_flight = 0;
         std::uint64_t last_done = 0;
+        std::size_t large_busy = 0;
         Status failure = Status::success;
         while (finished < count) {
             const std::uint64_t now = elapsed();
-            while (arrived < count && arrival_ns[arrived] <= now) ++arrived;
+            for (; arrived < count && arrival_ns[arrived] <= now; ++arrived)
+                queue[s.extents[arrived].bytes >= s.options.large_bytes ? 1 : 0].push_back(arrived);
             bool progress = false;
             for (Slot &slot : s.slots) {
                 if (!slot.busy) continue;
@@ -707,16 +718,28 @@ Status Server::replay(const ServeRequest *requests, const std::uint64_t *arrival
                 in_flight -= slot.bytes;
                 last_done = t;
                 slot.busy = false;
+                if (slot.large) --large_busy;
+                slot.large = false;
                 progress = true;
             }
             if (failure != Status::success) break;
             for (Slot &slot : s.slots) {
                 if (slot.busy || dispatched == arrived) continue;
-                // Admit arrived requests in order while the batch and the
+                // One class per batch: the older head, but large batches
+                // leave at least one stream to small requests.
+                const bool small_waiting = head[0] < queue[0].size(), large_waiting = head[1] < queue[1].size();
+                const bool large_room = s.slots.size() == 1 || large_busy + 1 < s.slots.size();
+                int q = 0;
+                if (small_waiting && large_waiting && large_room) q = queue[1][head[1]] < queue[0][head[0]] ? 1 : 0;
+                else if (!small_waiting && large_waiting && large_room) q = 1;
+                else if (!small_waiting) break;
+                // Admit that class's requests in order while the batch and the
                 // device have room; one request is always admitted alone.
+                const std::uint64_t *ids = queue[q].data() + head[q];
+                const std::uint64_t waiting = queue[q].size() - head[q];
                 std::uint64_t n = 0, tasks = 0, bytes = 0, staged = 0;
-                while (dispatched + n < arrived && n < s.options.max_batch) {
-                    const Extent &e = s.extents[dispatched + n];
+                while (n < waiting && n < s.options.max_batch) {
+                    const Extent &e = s.extents[ids[n]];
                     if (tasks + e.tasks > s.options.max_tasks) break;
                     if (staged + e.staged > s.options.staging_bytes) break;
                     if (in_flight + bytes + e.bytes > s.options.admission_bytes && (n != 0 || in_flight != 0)) break;
@@ -726,12 +749,15 @@ Status Server::replay(const ServeRequest *requests, const std::uint64_t *arrival
                     ++n;
                 }
                 if (n == 0) break;
-                if (const Status st = s.launch(slot, requests, dispatched, n); st != Status::success) {
+                if (const Status st = s.launch(slot, requests, ids, n, q == 1); st != Status::success) {
                     failure = st;
                     break;
                 }
                 slot.bytes = bytes;
+                slot.large = q == 1;
+                if (slot.large) ++large_busy;
                 in_flight += bytes;
+                head[q] += n;
                 dispatched += n;
                 ++stats.batches;
                 progress = true;
