@@ -739,11 +739,13 @@ def staged_leg(args, threads: int, stream):
 def file_leg(args, stream):
     """File in the page cache to calibrated FP32 on the device, per the
     reference's file tier (raw file: read, H2D, calibrate; .tsc file: read,
-    parse and plan, H2D, decode fused with (x - 100) * 1.25): a 4096x4096
-    uint16 detector frame and a 4096x4096 float32 field, 128x32 tiles.  Each
-    stage is timed on the host clock (device stages end with a stream
-    synchronize), median of the repetitions; the two results are compared
-    word for word."""
+    parse and plan, H2D, decode fused with (x - 100) * 1.25; TIFF file: read
+    and decode on one host thread, H2D from pageable memory, calibrate): a
+    4096x4096 uint16 detector frame and a 4096x4096 float32 field, 128x32
+    tiles (TIFF: Deflate level 6, the horizontal predictor for uint16 only).
+    Each stage is timed on the host clock (device stages end with a stream
+    synchronize), median of the repetitions; the results are compared word
+    for word."""
     import tempfile
     from tilestream_amd import api, generators
     side, tile = 4096, (128, 32)
@@ -759,6 +761,11 @@ def file_leg(args, stream):
             px.tofile(raw_path)
             with open(tsc_path, "wb") as f:
                 f.write(archive)
+            tiff_path = os.path.join(d, f"{kind}.tif")
+            assert api.write_tiff(tiff_path, px, api.tiff_write_options(
+                compression=1, horizontal_predictor=1 if kind == "u16" else 0, tile_width=tile[0],
+                tile_height=tile[1], deflate_level=6)) == api.Status.success
+            tiff_bytes = os.path.getsize(tiff_path)
             ptype = api.pixel_type_of(px.dtype)
             host = api.HostMemory(max(px.nbytes, len(archive)))
             hv = host.array()
@@ -827,19 +834,46 @@ def file_leg(args, stream):
                 tsc_t["parse_prepare"].append(t2 - t1)
                 tsc_t["h2d"].append(t3 - t2)
                 tsc_t["decode_calibrate"].append(t4 - t3)
-            same = bool(np.array_equal(out_raw.download(), out_tsc.download()))
+            tiff_t = {"read_decode": [], "h2d": [], "calibrate": []}
+            out_tiff = api.DeviceMemory(side * side * 4)
+            it3 = api.BatchItem()
+            it3.input = api.device_view(pixels, ptype, side, side, const=True)
+            it3.output = api.device_view(out_tiff, api.PixelType.float32, side, side)
+            it3.calibration.offset = api.scalar(offset)
+            it3.calibration.gain = api.scalar(gain)
+            st, tiff_plan = api.prepare_batch(api.Operation.calibrate, [it3], stream=stream)
+            assert st == api.Status.success, st
+            for _ in range(reps):
+                t0 = time.perf_counter()
+                st, decoded = api.read_tiff(tiff_path)
+                assert st == api.Status.success, st
+                t1 = time.perf_counter()
+                pixels.upload(decoded)  # from pageable memory, as the reference's TiffImage
+                t2 = time.perf_counter()
+                assert api.calibrate_async(tiff_plan, ws, res, 1, stream) == api.Status.success
+                stream.synchronize()
+                t3 = time.perf_counter()
+                tiff_t["read_decode"].append(t1 - t0)
+                tiff_t["h2d"].append(t2 - t1)
+                tiff_t["calibrate"].append(t3 - t2)
+            same = bool(np.array_equal(out_raw.download(), out_tsc.download())) and bool(
+                np.array_equal(out_raw.download(), out_tiff.download()))
             med = lambda v: round(float(np.median(v)) * 1e3, 3)  # noqa: E731
             rec = {"ratio": round(px.nbytes / len(archive), 3),
+                   "tiff_ratio": round(px.nbytes / tiff_bytes, 3),
                    "raw_file": {k: med(v) for k, v in raw_t.items()},
                    "tsc_file": {k: med(v) for k, v in tsc_t.items()},
+                   "tiff_file": {k: med(v) for k, v in tiff_t.items()},
                    "identical_outputs": same}
-            rec["raw_file"]["host_end_to_end"] = round(sum(rec["raw_file"].values()), 3)
-            rec["tsc_file"]["host_end_to_end"] = round(sum(rec["tsc_file"].values()), 3)
+            for tier in ("raw_file", "tsc_file", "tiff_file"):
+                rec[tier]["host_end_to_end"] = round(sum(rec[tier].values()), 3)
             log(f"file tier {kind}: raw {rec['raw_file']['host_end_to_end']} ms, tsc {rec['tsc_file']['host_end_to_end']} "
-                f"ms ({rec['tsc_file']}), identical {same}")
+                f"ms ({rec['tsc_file']}), tiff {rec['tiff_file']['host_end_to_end']} ms (ratio {rec['tiff_ratio']}), "
+                f"identical {same}")
             out[kind] = rec
     return {"workload": "4096x4096 uint16 frame and float32 field, 128x32 tiles, file in the page cache to FP32 "
-                        "(x - 100) * 1.25 on the device; stage medians in ms", **out,
+                        "(x - 100) * 1.25 on the device; raw, .tsc and TIFF (Deflate 6) files; stage medians in ms",
+            **out,
             "verified_bit_exact": all(r["identical_outputs"] for r in out.values())}
 
 

@@ -13,6 +13,7 @@
  *                          <- reference include/tilestream/sizing.h:26-30
  *   ts_parse_metadata      <- reference include/tilestream/api.h:42-43
  *   ts_cpu_compress / ts_cpu_decompress <- reference include/tilestream/cpu.h:9-12
+ *   ts_read_tiff / ts_write_tiff <- reference include/tilestream/tiff.h:37-40
  *   ts_prepare_batch       <- reference include/tilestream/cuda.h:176-177
  *   ts_plan_workspace_spec <- reference include/tilestream/cuda.h:156
  *   ts_compress_async, ts_decompress_async, ts_decode_roi_async,
@@ -94,6 +95,28 @@ uint32_t ts_metadata_set_segment(ts_metadata *metadata, uint64_t index, const ts
 uint32_t ts_cpu_compress(ts_const_image_view input, ts_tile_shape tile, ts_mutable_bytes output,
                          uint64_t *bytes_written);
 uint32_t ts_cpu_decompress(ts_const_bytes input, ts_image_view output, const ts_parse_limits *limits);
+
+/* ---- TIFF files (one uint16 or float32 sample per pixel) ----------------- */
+typedef struct ts_tiff_image ts_tiff_image;
+typedef struct {
+    uint32_t max_width, max_height;
+    uint64_t max_decoded_bytes;
+} ts_tiff_read_limits;
+typedef struct {
+    uint32_t bigtiff;              /* 0: classic TIFF, 1: BigTIFF */
+    uint32_t compression;          /* 0: none, 1: Deflate */
+    uint32_t horizontal_predictor; /* uint16 with Deflate only */
+    uint32_t tile_width, tile_height;
+    uint32_t deflate_level;
+} ts_tiff_write_options;
+void ts_tiff_default_options(ts_tiff_read_limits *limits, ts_tiff_write_options *options);
+/* On success *out owns the decoded image (free with ts_tiff_image_free); limits may be NULL. */
+uint32_t ts_read_tiff(const char *path, const ts_tiff_read_limits *limits, ts_tiff_image **out);
+/* The image, its row pitch (packed rows) and its pixels (host memory owned by `image`). */
+void ts_tiff_image_get(const ts_tiff_image *image, ts_image_spec *spec, uint64_t *pitch_bytes,
+                       const void **pixels);
+void ts_tiff_image_free(ts_tiff_image *image);
+uint32_t ts_write_tiff(const char *path, ts_const_image_view image, const ts_tiff_write_options *options);
 
 /* ---- GPU plans and execution ------------------------------------------- */
 /* On success *plan is replaced (the previous plan, if any, is destroyed). */

@@ -2,12 +2,15 @@
 // test_host_sanitizers.py with g++): the CPU encoder, the parser (several MiB
 // per archive, so its validation runs on several threads), the CPU decoder,
 // and parses of archives with one flipped bit in the index or the payloads,
-// which must fail cleanly or succeed.
+// which must fail cleanly or succeed; TIFF export and import of the pixels.
 #include <tilestream/api.h>
 #include <tilestream/cpu.h>
 #include <tilestream/sizing.h>
+#include <tilestream/tiff.h>
 
 #include <cstdio>
+#include <cstdlib>
+#include <string>
 #include <cstring>
 #include <random>
 #include <vector>
@@ -46,9 +49,33 @@ static int run(PixelType type, std::uint32_t w, std::uint32_t h, TileShape tile,
         ImageMetadata m2;
         if (parse_metadata(ConstBytes{bad.data(), bad.size()}, ParseLimits{}, m2) != Status::success) ++failures;
     }
-    std::printf("%s %ux%u tile %ux%u: %llu bytes, %d of 64 corruptions rejected\n",
+    // TIFF export and import of the decoded pixels (Deflate, predictor for uint16),
+    // then a copy with one flipped bit must fail cleanly or succeed.
+    const char *dir = std::getenv("TMPDIR");
+    const std::string path = std::string(dir != nullptr ? dir : "/tmp") + "/host_sanitize_" + std::to_string(seed) + ".tif";
+    io::TiffWriteOptions options;
+    options.compression = io::TiffCompression::deflate;
+    options.horizontal_predictor = type == PixelType::uint16;
+    if (io::write_tiff(path.c_str(), ConstImageView{out.data(), spec, std::uint64_t{w} * s, out.size()}, options) !=
+        Status::success)
+        return 6;
+    io::TiffImage tiff;
+    if (io::read_tiff(path.c_str(), tiff) != Status::success || tiff.pixels != px) return 7;
+    std::FILE *f = std::fopen(path.c_str(), "r+b");
+    if (f == nullptr) return 8;
+    std::fseek(f, 0, SEEK_END);
+    const long tiff_bytes = std::ftell(f);
+    std::fseek(f, tiff_bytes / 2, SEEK_SET);
+    const int byte = std::fgetc(f);
+    std::fseek(f, tiff_bytes / 2, SEEK_SET);
+    std::fputc(byte ^ 0x10, f);
+    std::fclose(f);
+    io::TiffImage broken;
+    const Status tiff_status = io::read_tiff(path.c_str(), broken);
+    std::remove(path.c_str());
+    std::printf("%s %ux%u tile %ux%u: %llu bytes, %d of 64 corruptions rejected; corrupted TIFF status %u\n",
                 type == PixelType::uint16 ? "u16" : "f32", w, h, tile.width, tile.height,
-                static_cast<unsigned long long>(written), failures);
+                static_cast<unsigned long long>(written), failures, static_cast<unsigned>(tiff_status));
     return 0;
 }
 

@@ -1,6 +1,7 @@
-"""The host parser and CPU codec under ThreadSanitizer and AddressSanitizer +
-UndefinedBehaviorSanitizer: tests/host_sanitize.cpp built with g++ against the
-product's host sources, run on archives large enough for the threaded parse."""
+"""The host parser, CPU codec and TIFF I/O under ThreadSanitizer and
+AddressSanitizer + UndefinedBehaviorSanitizer: tests/host_sanitize.cpp built
+with g++ against the product's host sources, run on archives large enough for
+the threaded parse, and the TIFF export and import of the same pixels."""
 import os
 import shutil
 import subprocess
@@ -10,7 +11,7 @@ import pytest
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 HOST = os.path.join(ROOT, "tilestream_amd", "csrc", "host")
 SOURCES = [os.path.join(ROOT, "tests", "host_sanitize.cpp")] + [
-    os.path.join(HOST, f) for f in ("parser.cpp", "cpu_codec.cpp", "sizing.cpp")]
+    os.path.join(HOST, f) for f in ("parser.cpp", "cpu_codec.cpp", "sizing.cpp", "tiff.cpp")]
 
 
 @pytest.mark.parametrize("sanitizer", ["thread", "address,undefined"])
@@ -20,7 +21,7 @@ def test_host_code_under_sanitizer(tmp_path, sanitizer):
     exe = str(tmp_path / "host_sanitize")
     build = subprocess.run(["g++", "-std=c++17", "-O1", "-g", "-fno-omit-frame-pointer", "-pthread",
                             f"-fsanitize={sanitizer}", "-fno-sanitize-recover=all",
-                            "-I" + os.path.join(ROOT, "include"), "-I" + HOST] + SOURCES + ["-o", exe],
+                            "-I" + os.path.join(ROOT, "include"), "-I" + HOST] + SOURCES + ["-o", exe, "-lz"],
                            capture_output=True, text=True, timeout=300)
     assert build.returncode == 0, build.stderr[-3000:]
     env = dict(os.environ, TSAN_OPTIONS="halt_on_error=1", ASAN_OPTIONS="detect_leaks=1:halt_on_error=1")

@@ -146,6 +146,15 @@ class ServerOptions(C.Structure):
                 ("shape", LaunchShape), ("large_bytes", C.c_uint64)]
 
 
+class TiffReadLimits(C.Structure):
+    _fields_ = [("max_width", C.c_uint32), ("max_height", C.c_uint32), ("max_decoded_bytes", C.c_uint64)]
+
+
+class TiffWriteOptions(C.Structure):
+    _fields_ = [("bigtiff", C.c_uint32), ("compression", C.c_uint32), ("horizontal_predictor", C.c_uint32),
+                ("tile_width", C.c_uint32), ("tile_height", C.c_uint32), ("deflate_level", C.c_uint32)]
+
+
 class ServerStats(C.Structure):
     _fields_ = [("requests", C.c_uint64), ("batches", C.c_uint64), ("decoded_bytes", C.c_uint64),
                 ("failed", C.c_uint64), ("seconds", C.c_double)]
@@ -162,7 +171,8 @@ RECORDS = {
     "ts_compressed_input": CompressedInput, "ts_batch_item": BatchItem,
     "ts_batch_request": BatchRequest, "ts_launch_shape": LaunchShape, "ts_queue_stats": QueueStats,
     "ts_archive_info": ArchiveInfo, "ts_serve_request": ServeRequest, "ts_server_options": ServerOptions,
-    "ts_server_stats": ServerStats,
+    "ts_server_stats": ServerStats, "ts_tiff_read_limits": TiffReadLimits,
+    "ts_tiff_write_options": TiffWriteOptions,
 }
 
 _u32, _u64, _vp, _int = C.c_uint32, C.c_uint64, C.c_void_p, C.c_int
@@ -182,6 +192,11 @@ PROTOTYPES = {
     "ts_metadata_set_segment": (_u32, [_vp, _u64, _P(SegmentMetadata)]),
     "ts_cpu_compress": (_u32, [ConstImageView, TileShape, MutableBytes, _P(_u64)]),
     "ts_cpu_decompress": (_u32, [ConstBytes, ImageView, _P(ParseLimits)]),
+    "ts_tiff_default_options": (None, [_P(TiffReadLimits), _P(TiffWriteOptions)]),
+    "ts_read_tiff": (_u32, [C.c_char_p, _P(TiffReadLimits), _P(_vp)]),
+    "ts_tiff_image_get": (None, [_vp, _P(ImageSpec), _P(_u64), _P(_vp)]),
+    "ts_tiff_image_free": (None, [_vp]),
+    "ts_write_tiff": (_u32, [C.c_char_p, ConstImageView, _P(TiffWriteOptions)]),
     "ts_prepare_batch": (_u32, [_P(BatchRequest), _P(_vp), _vp]),
     "ts_plan_destroy": (None, [_vp]),
     "ts_plan_workspace_spec": (None, [_vp, _P(WorkspaceSpec)]),

@@ -9,6 +9,7 @@ host side.
 from __future__ import annotations
 
 import ctypes as C
+import os
 from enum import IntEnum
 from typing import Optional, Sequence
 
@@ -186,6 +187,45 @@ def cpu_decompress(data, out: np.ndarray, limits: Optional[ParseLimits] = None) 
     arr, cb = _host_bytes(data)
     lim = limits or ParseLimits(MAX_BYTES, MAX_BYTES, MAX_BYTES)
     return Status(N.lib().ts_cpu_decompress(cb, host_view(out, const=False), C.byref(lim)))
+
+
+# ---- TIFF files ------------------------------------------------------------------
+
+def tiff_write_options(**fields) -> "N.TiffWriteOptions":
+    """The library's default TIFF write options with the given fields replaced
+    (bigtiff, compression 0 none / 1 Deflate, horizontal_predictor, tile_width,
+    tile_height, deflate_level)."""
+    o = N.TiffWriteOptions()
+    N.lib().ts_tiff_default_options(None, C.byref(o))
+    for k, v in fields.items():
+        setattr(o, k, int(v))
+    return o
+
+
+def write_tiff(path: str, pixels: np.ndarray, options: "Optional[N.TiffWriteOptions]" = None) -> Status:
+    """Writes a uint16 or float32 (stored as its bit pattern) 2-D image (reference io::write_tiff)."""
+    opts = options if options is not None else tiff_write_options()
+    return Status(N.lib().ts_write_tiff(os.fsencode(path), host_view(pixels), C.byref(opts)))
+
+
+def read_tiff(path: str, limits: "Optional[N.TiffReadLimits]" = None):
+    """(status, pixels): the decoded image as uint16, or float32 samples as uint32
+    bit patterns (reference io::read_tiff)."""
+    handle = C.c_void_p()
+    st = Status(N.lib().ts_read_tiff(os.fsencode(path), C.byref(limits) if limits is not None else None,
+                                     C.byref(handle)))
+    if st != Status.success:
+        return st, None
+    try:
+        spec, pitch, data = ImageSpec(), C.c_uint64(), C.c_void_p()
+        N.lib().ts_tiff_image_get(handle, C.byref(spec), C.byref(pitch), C.byref(data))
+        dtype = np.uint16 if spec.type == int(PixelType.uint16) else np.uint32
+        out = np.empty((spec.height, spec.width), dtype)
+        if out.nbytes:
+            C.memmove(out.ctypes.data, data.value, out.nbytes)
+        return st, out
+    finally:
+        N.lib().ts_tiff_image_free(handle)
 
 
 # ---- device memory, streams, events --------------------------------------------

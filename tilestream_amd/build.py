@@ -33,7 +33,7 @@ HOST_FLAGS = ["-x", "c++", "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include", "-Wa
 
 HIP_SOURCES = ["hip/decode_mixed.hip", "hip/decode_u16.hip", "hip/decode_f32.hip", "hip/decode_u16_cal.hip",
                "hip/decode_f32_cal.hip", "hip/decode.hip", "hip/encode.hip", "hip/calibrate.hip"]
-HOST_SOURCES = ["host/sizing.cpp", "host/parser.cpp", "host/cpu_codec.cpp", "host/plan.cpp",
+HOST_SOURCES = ["host/sizing.cpp", "host/parser.cpp", "host/cpu_codec.cpp", "host/tiff.cpp", "host/plan.cpp",
                 "host/queue.cpp", "host/serve.cpp", "host/capi.cpp"]
 
 
@@ -66,7 +66,7 @@ def build(verbose: bool = True) -> None:
         objs = list(pool.map(_compile, sources))
     lib = os.path.join(OUT, "libtilestream_amd.so")
     tmp = lib + ".tmp"
-    subprocess.run([HIPCC, "-shared", "-fPIC", "--offload-arch=" + ARCH, "-o", tmp] + objs,
+    subprocess.run([HIPCC, "-shared", "-fPIC", "--offload-arch=" + ARCH, "-o", tmp] + objs + ["-lz"],
                    check=True)
     os.replace(tmp, lib)
     gen = os.path.join(OUT, "libtsgen.so")

@@ -7,6 +7,7 @@
 #include <tilestream/cpu.h>
 #include <tilestream/hip.h>
 #include <tilestream/serve.h>
+#include <tilestream/tiff.h>
 #include <tilestream_c.h>
 
 #include <cstddef>
@@ -130,6 +131,55 @@ uint32_t ts_parse_metadata(ts_const_bytes source, const ts_parse_limits *limits,
 }
 
 void ts_metadata_free(ts_metadata *m) { delete meta(m); }
+
+void ts_tiff_default_options(ts_tiff_read_limits *limits, ts_tiff_write_options *options) {
+    if (limits != nullptr) {
+        const io::TiffReadLimits l;
+        *limits = {l.max_width, l.max_height, l.max_decoded_bytes};
+    }
+    if (options != nullptr) {
+        const io::TiffWriteOptions o;
+        *options = {o.bigtiff ? 1u : 0u, static_cast<uint32_t>(o.compression), o.horizontal_predictor ? 1u : 0u,
+                    o.tile_width, o.tile_height, o.deflate_level};
+    }
+}
+
+uint32_t ts_read_tiff(const char *path, const ts_tiff_read_limits *limits, ts_tiff_image **out) {
+    if (out == nullptr) return code(Status::invalid_argument);
+    try {
+        auto image = std::make_unique<io::TiffImage>();
+        io::TiffReadLimits l;
+        if (limits != nullptr) l = {limits->max_width, limits->max_height, limits->max_decoded_bytes};
+        const Status st = io::read_tiff(path, *image, l);
+        if (st == Status::success) *out = reinterpret_cast<ts_tiff_image *>(image.release());
+        return code(st);
+    } catch (const std::bad_alloc &) {
+        return code(Status::resource_limit);
+    }
+}
+
+void ts_tiff_image_get(const ts_tiff_image *image, ts_image_spec *spec, uint64_t *pitch_bytes, const void **pixels) {
+    const auto *t = reinterpret_cast<const io::TiffImage *>(image);
+    if (spec != nullptr) *spec = as<ts_image_spec>(t->image);
+    if (pitch_bytes != nullptr) *pitch_bytes = t->pitch_bytes;
+    if (pixels != nullptr) *pixels = t->pixels.data();
+}
+
+void ts_tiff_image_free(ts_tiff_image *image) { delete reinterpret_cast<io::TiffImage *>(image); }
+
+uint32_t ts_write_tiff(const char *path, ts_const_image_view image, const ts_tiff_write_options *options) {
+    io::TiffWriteOptions o;
+    if (options != nullptr) {
+        if (options->compression > 1) return code(Status::invalid_argument);
+        o.bigtiff = options->bigtiff != 0;
+        o.compression = static_cast<io::TiffCompression>(options->compression);
+        o.horizontal_predictor = options->horizontal_predictor != 0;
+        o.tile_width = options->tile_width;
+        o.tile_height = options->tile_height;
+        o.deflate_level = options->deflate_level;
+    }
+    return code(io::write_tiff(path, as<ConstImageView>(image), o));
+}
 
 void ts_metadata_get_summary(const ts_metadata *m, ts_metadata_summary *out) {
     const ImageMetadata &x = *meta(m);
@@ -446,6 +496,8 @@ uint64_t ts_abi_sizeof(const char *name) {
     TS_SIZE(ts_serve_request)
     TS_SIZE(ts_server_options)
     TS_SIZE(ts_server_stats)
+    TS_SIZE(ts_tiff_read_limits)
+    TS_SIZE(ts_tiff_write_options)
 #undef TS_SIZE
     return 0;
 }
