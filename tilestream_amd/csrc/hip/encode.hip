@@ -280,12 +280,12 @@ __global__ void __launch_bounds__(Cfg<U16>::threads) __attribute__((amdgpu_waves
     std::uint32_t any = 0, seed = 0;
     // Residuals stay in registers until the packing; uint16 residuals (at
     // most 16 bits) two to a register.  The sample before a lane's first
-    // column is the previous lane's last.
+    // column is the previous lane's last; column 0 is taken against itself
+    // (residual 0, never packed: the row's seed stands for it).
     constexpr std::uint32_t kPer = U16 ? 2u : 1u;
     std::uint32_t z[kColumns / kPer];
 #pragma unroll
     for (std::uint32_t i = 0; i < kColumns / kPer; ++i) z[i] = 0;
-    // (Column 0 gets residual 0 against itself.)
     const bool full = r < g.h && c0 + kColumns <= g.w;
     if (r < g.h && c0 < g.w) {
         const Sample *lane_px = px + place<U16>(r, c0, row);
