@@ -197,3 +197,17 @@ def test_invalid_writes(tmp_path):
         assert not os.path.exists(path)
     # A path that cannot be created: an I/O error, no file left behind.
     assert api.write_tiff(str(tmp_path / "no-such-dir" / "x.tif"), px) == Status.io_error
+
+
+def test_large_files_decode_on_several_threads(tmp_path):
+    """Files of several MiB split their tiles or strips over host threads; the
+    result is the same image as one thread gives."""
+    px = _u16(2048, 2304, 11)  # 9 MiB of samples
+    strips = str(tmp_path / "strips.tif")
+    Image.fromarray(px).save(strips)  # Pillow: uncompressed strips
+    st, back = api.read_tiff(strips)
+    assert st == Status.success and np.array_equal(back, px)
+    tiles = str(tmp_path / "tiles.tif")
+    assert api.write_tiff(tiles, px, api.tiff_write_options(tile_width=64, tile_height=64)) == Status.success
+    st, back = api.read_tiff(tiles)
+    assert st == Status.success and np.array_equal(back, px)

@@ -208,6 +208,42 @@ void decode_chunk(const TiffFile &t, std::uint64_t offset, std::uint64_t count, 
     }
 }
 
+// Runs body(begin, end) over contiguous ranges of [0, n) on up to 16 host
+// threads (one per 4 MiB of file: tiles and strips are independent); returns
+// the status of a failing range (every chunk failure is io_error).
+template <typename Body> Status parallel_ranges(std::uint64_t n, std::uint64_t file_bytes, const Body &body) {
+    auto run = [&body](std::uint64_t begin, std::uint64_t end, Status &status) {
+        try {
+            body(begin, end);
+        } catch (const Malformed &) {
+            status = Status::io_error;
+        } catch (const std::bad_alloc &) {
+            status = Status::resource_limit;
+        } catch (const std::length_error &) {
+            status = Status::resource_limit;
+        }
+    };
+    const std::uint64_t hw = std::max(1u, std::thread::hardware_concurrency());
+    const std::uint64_t threads = std::max<std::uint64_t>(1, std::min<std::uint64_t>({hw, 16, file_bytes / (4u << 20) + 1, n}));
+    std::vector<Status> status(static_cast<std::size_t>(threads), Status::success);
+    if (threads == 1) {
+        run(0, n, status[0]);
+    } else {
+        std::vector<std::thread> pool;
+        try {
+            for (std::uint64_t j = 0; j < threads; ++j)
+                pool.emplace_back(run, n * j / threads, n * (j + 1) / threads, std::ref(status[j]));
+        } catch (const std::system_error &) {
+            for (std::thread &worker : pool) worker.join();
+            return Status::resource_limit;
+        }
+        for (std::thread &worker : pool) worker.join();
+    }
+    for (Status st : status)
+        if (st != Status::success) return st;
+    return Status::success;
+}
+
 Status decode(const TiffFile &t, const TiffReadLimits &limits, TiffImage &img) {
     const std::uint64_t width = t.scalar(kWidth, 0, true), height = t.scalar(kLength, 0, true);
     const std::uint64_t samples = t.scalar(kSamples, 1), bits = t.scalar(kBits, 1), format = t.scalar(kSampleFormat, 1);
@@ -247,45 +283,18 @@ Status decode(const TiffFile &t, const TiffReadLimits &limits, TiffImage &img) {
         const std::vector<std::uint64_t> offsets = t.values(*oe), counts = t.values(*ce);
         if (offsets.size() != tiles || counts.size() != tiles) return Status::unsupported_format;
         out.pixels.resize(static_cast<std::size_t>(total));
-        // Tiles are independent: contiguous tile ranges over up to 16 host
-        // threads for files of several MiB (a thread per 4 MiB of file).
-        auto run = [&](std::uint64_t begin, std::uint64_t end, Status &status) {
-            try {
-                std::vector<std::uint8_t> chunk(static_cast<std::size_t>(tile_bytes));
-                for (std::uint64_t k = begin; k < end; ++k) {
-                    decode_chunk(t, offsets[k], counts[k], deflate, horizontal, sample, tw, chunk);
-                    const std::uint64_t x = (k % across) * tw, y = (k / across) * th;
-                    const std::uint64_t cw = std::min(tw, width - x), ch = std::min(th, height - y);
-                    for (std::uint64_t r = 0; r < ch; ++r)
-                        std::memcpy(out.pixels.data() + (y + r) * row_bytes + x * sample,
-                                    chunk.data() + r * tw * sample, static_cast<std::size_t>(cw * sample));
-                }
-            } catch (const Malformed &) {
-                status = Status::io_error;
-            } catch (const std::bad_alloc &) {
-                status = Status::resource_limit;
-            } catch (const std::length_error &) {
-                status = Status::resource_limit;
+        const Status st = parallel_ranges(tiles, t.size(), [&](std::uint64_t begin, std::uint64_t end) {
+            std::vector<std::uint8_t> chunk(static_cast<std::size_t>(tile_bytes));
+            for (std::uint64_t k = begin; k < end; ++k) {
+                decode_chunk(t, offsets[k], counts[k], deflate, horizontal, sample, tw, chunk);
+                const std::uint64_t x = (k % across) * tw, y = (k / across) * th;
+                const std::uint64_t cw = std::min(tw, width - x), ch = std::min(th, height - y);
+                for (std::uint64_t r = 0; r < ch; ++r)
+                    std::memcpy(out.pixels.data() + (y + r) * row_bytes + x * sample,
+                                chunk.data() + r * tw * sample, static_cast<std::size_t>(cw * sample));
             }
-        };
-        const std::uint64_t hw = std::max(1u, std::thread::hardware_concurrency());
-        const std::uint64_t threads = std::min<std::uint64_t>({hw, 16, t.size() / (4u << 20) + 1, tiles});
-        std::vector<Status> status(static_cast<std::size_t>(threads), Status::success);
-        if (threads <= 1) {
-            run(0, tiles, status[0]);
-        } else {
-            std::vector<std::thread> pool;
-            try {
-                for (std::uint64_t j = 0; j < threads; ++j)
-                    pool.emplace_back(run, tiles * j / threads, tiles * (j + 1) / threads, std::ref(status[j]));
-            } catch (const std::system_error &) {
-                for (std::thread &th_ : pool) th_.join();
-                return Status::resource_limit;
-            }
-            for (std::thread &th_ : pool) th_.join();
-        }
-        for (Status s : status)
-            if (s != Status::success) return s;
+        });
+        if (st != Status::success) return st;
     } else {
         const std::uint64_t rps = std::min<std::uint64_t>(t.scalar(kRowsPerStrip, UINT32_MAX), height);
         if (rps == 0) return Status::unsupported_format;
@@ -295,13 +304,16 @@ Status decode(const TiffFile &t, const TiffReadLimits &limits, TiffImage &img) {
         const std::vector<std::uint64_t> offsets = t.values(*oe), counts = t.values(*ce);
         if (offsets.size() != strips || counts.size() != strips) return Status::unsupported_format;
         out.pixels.resize(static_cast<std::size_t>(total));
-        std::vector<std::uint8_t> chunk;
-        for (std::uint64_t s = 0; s < strips; ++s) {
-            const std::uint64_t rows = std::min(rps, height - s * rps);
-            chunk.resize(static_cast<std::size_t>(rows * row_bytes));
-            decode_chunk(t, offsets[s], counts[s], deflate, horizontal, sample, width, chunk);
-            std::memcpy(out.pixels.data() + s * rps * row_bytes, chunk.data(), chunk.size());
-        }
+        const Status st = parallel_ranges(strips, t.size(), [&](std::uint64_t begin, std::uint64_t end) {
+            std::vector<std::uint8_t> chunk;
+            for (std::uint64_t k = begin; k < end; ++k) {
+                const std::uint64_t rows = std::min(rps, height - k * rps);
+                chunk.resize(static_cast<std::size_t>(rows * row_bytes));
+                decode_chunk(t, offsets[k], counts[k], deflate, horizontal, sample, width, chunk);
+                std::memcpy(out.pixels.data() + k * rps * row_bytes, chunk.data(), chunk.size());
+            }
+        });
+        if (st != Status::success) return st;
     }
     img = std::move(out);
     return Status::success;
