@@ -42,6 +42,19 @@ def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+def sampled(stream, fn, samples: int = 30, min_ms: float = 2.0):
+    """The reference's sample rule (benchmarks/runner.h:197-246) in one process:
+    each sample repeats fn until it lasts at least min_ms (at most 1000 calls),
+    timed with events on the work's stream; nearest-rank p05/p50/p95 of the
+    per-call times in ms."""
+    one = timed(stream, fn, 1)
+    reps = int(min(1000, max(1, np.ceil(min_ms / max(one, 1e-6)))))
+    per_call = sorted(timed(stream, fn, reps) for _ in range(samples))
+    rank = lambda p: per_call[max(0, int(np.ceil(p * len(per_call))) - 1)]  # noqa: E731
+    return {"samples": samples, "calls_per_sample": reps, "p05_ms": round(rank(0.05), 4),
+            "p50_ms": round(rank(0.50), 4), "p95_ms": round(rank(0.95), 4)}
+
+
 def timed(stream, fn, steps: int):
     from tilestream_amd import api
     start, stop = api.Event(), api.Event()
@@ -1193,6 +1206,13 @@ def main() -> int:
     for _ in range(args.warmup):
         per_image()
     ms_plain = timed(stream, per_image, max(3, args.steps // 4))
+    protocol = {"batched": sampled(stream, launch), "per_image": sampled(stream, per_image),
+                "note": "reference sample rule: 30 samples of at least 2 ms, events on the stream, nearest-rank "
+                        "percentiles of the per-launch time (one process; the reference also repeats the "
+                        "process 5 times)"}
+    protocol["speedup_p50"] = round(protocol["per_image"]["p50_ms"] / protocol["batched"]["p50_ms"], 3)
+    log(f"protocol: batched p50 {protocol['batched']['p50_ms']} ms, per-image p50 {protocol['per_image']['p50_ms']} "
+        f"ms, speedup {protocol['speedup_p50']}")
 
     # ---- verification --------------------------------------------------------------
     verified = None
@@ -1251,6 +1271,7 @@ def main() -> int:
         "per_image": {"value": round(gbps_plain, 2), "unit": "GB/s", "ms_per_step": round(ms_plain, 4),
                       "launches_per_step": 3 * n},
         "speedup_vs_per_image": round(ms_plain / ms_batched, 3),
+        "timing_protocol": protocol,
         "roofline": {
             "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": traffic,
