@@ -916,6 +916,35 @@ def fp64_errors(got_bits: np.ndarray, x: np.ndarray, offset: np.ndarray, gain: n
             "nonfinite_class_mismatches": nan_mismatch + inf_mismatch, "pixels": int(x.size)}
 
 
+def fused_cpu_baseline(archive, px, dark, gain, gpu_bits):
+    """The fused workload on one host thread, as the reference measures its CPU
+    path for fused configs: the reference CPU codec decodes the archive
+    (oracle/_ref; the C oracle where it is absent), then host binary32
+    (x - offset) * gain with the same maps (two roundings, as on the GPU)."""
+    from oracle import oracle
+    ref = oracle.reference() is not None
+    h, w = px.shape
+    ptype = 1 if px.dtype == np.uint16 else 2
+    reps, best = 3, None
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        dec = oracle.ref_decompress(archive, ptype, w, h) if ref else oracle.decompress(archive)
+        x = dec.astype(np.float32) if dec.dtype == np.uint16 else dec.view(np.float32)
+        with np.errstate(invalid="ignore", over="ignore"):
+            res = (x - dark) * gain
+        t = time.perf_counter() - t0
+        best = t if best is None else min(best, t)
+    host_bits = res.astype(np.float32).view(np.uint32).reshape(-1)
+    gpu_bits = np.asarray(gpu_bits).view(np.uint32).reshape(-1)
+    nan_host, nan_gpu = np.isnan(host_bits.view(np.float32)), np.isnan(gpu_bits.view(np.float32))
+    same = bool(np.array_equal(nan_host, nan_gpu) and np.array_equal(host_bits[~nan_host], gpu_bits[~nan_host]))
+    return {"ms": round(best * 1e3, 2), "decoded_GBps": round(px.nbytes / best / 1e9, 3), "cores": 1,
+            "kind": "reference" if ref else "port",
+            "equals_gpu": same, "equality": "word for word, NaN by class",
+            "sample": "whole decode of the 4096x4096 archive + host binary32 calibration with both maps, "
+                      "best of 3 on one thread"}
+
+
 def fused_leg(args, stream):
     """Plain method at real shape: one 4096x4096 uint16 detector frame and one
     4096x4096 float32 field (64x64 tiles), whole decode fused with a per-pixel
@@ -972,6 +1001,8 @@ def fused_leg(args, stream):
             "algorithmic_bytes": alg, "status": int(status[0]),
             "fp64_error": fp64_errors(got, px, dark, gain),
         }
+        if not args.no_cpu:
+            out[kind]["cpu_baseline"] = fused_cpu_baseline(archive, px, dark, gain, got)
         log(f"fused {kind}: {ms:.4f} ms, {out[kind]['algorithmic_GBps']} GB/s algorithmic, "
             f"max {out[kind]['fp64_error']['max_ulp']} ulp")
         # Standalone calibration of the raw image with the same maps (reference
