@@ -325,6 +325,39 @@ def roi_leg(args, threads: int, stream):
     }
 
 
+def encode_cpu_baseline(batch, threads: int, sample_bytes: int):
+    """The reference CPU encoder (oracle/_ref; the C oracle where it is absent)
+    on the first images of the batch up to about sample_bytes of pixels: one
+    thread, then `threads` threads with one image each; the archives must equal
+    the batch's."""
+    from oracle import oracle
+    ref = oracle.reference() is not None
+    enc = oracle.ref_compress if ref else oracle.compress
+    sample, acc = [], 0
+    for e in batch.entries:
+        if acc >= sample_bytes:
+            break
+        sample.append(e)
+        acc += e.raw_bytes
+
+    def one(e):
+        tile = (e.tile.width, e.tile.height) if hasattr(e.tile, "width") else tuple(e.tile)
+        data = enc(e.pixels, tile)
+        return bytes(data) == bytes(e.archive)
+
+    t0 = time.perf_counter()
+    same = all(one(e) for e in sample)
+    single = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    with cf.ThreadPoolExecutor(threads) as pool:
+        same = all(pool.map(one, sample)) and same
+    multi = time.perf_counter() - t0
+    return {"value": round(acc / multi / 1e9, 3), "unit": "GB/s of pixels", "cores": threads,
+            "kind": "reference" if ref else "port", "single_thread_GBps": round(acc / single / 1e9, 3),
+            "identical_archives": same,
+            "sample": f"first {len(sample)} images of the batch ({acc / 1e9:.2f} GB of pixels), one image per thread"}
+
+
 def encode_leg(args, batch, stream):
     """GPU canonical encode of the decode batch's images (pixels resident in
     HBM): one compress plan per (pixel type, tile shape), all six run back to
@@ -391,6 +424,7 @@ def encode_leg(args, batch, stream):
             ok = ok and out.download(len(batch.entries[i].archive), off).tobytes() == batch.entries[i].archive
     raw_bytes = batch.decoded_bytes
     alg = raw_bytes + written_total
+    cpu = None if args.no_cpu else encode_cpu_baseline(batch, 16, 500_000_000)
     log(f"encode: {ms:.4f} ms per step, {raw_bytes / (ms * 1e-3) / 1e9:.1f} GB/s of pixels, "
         f"{alg / (ms * 1e-3) / 1e9:.1f} GB/s algorithmic, verified {ok}")
     default_workload = args.images == 256 and args.seed == 2026517
@@ -404,6 +438,7 @@ def encode_leg(args, batch, stream):
                          if default_workload else None,
                          "algorithmic_bytes_per_step": alg},
             "archive_bytes": written_total,
+            "cpu_baseline": cpu,
             "note": "algorithmic bytes = pixels read once + archive bytes written; traffic = HBM bytes per step "
                     "from rocprofv3 FETCH_SIZE/WRITE_SIZE (profiles/encode_batch256_kernel_trace.md)",
             "verified_bit_exact": ok}
