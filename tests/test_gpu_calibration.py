@@ -171,3 +171,20 @@ def test_contract_rows_fused_with_decode(orc):
     assert status[0] == 0 and written[0] == x.size * 4
     ok, bad = same_or_both_nan(outs[0], want)
     assert ok, bad
+
+
+@pytest.mark.parametrize("kind", ["u16", "f32"])
+def test_tiff_file_to_calibrated_output(orc, tmp_path, kind):
+    """The file tier's TIFF path: pixels exported as a Deflate TIFF, imported
+    on the host and calibrated on the GPU equal the host binary32 calibration
+    of the original pixels word for word (NaN by class)."""
+    px = g.detector_u16(300, 211, 41) if kind == "u16" else g.field_f32(300, 211, 41)
+    path = str(tmp_path / "f.tif")
+    opts = api.tiff_write_options(compression=1, horizontal_predictor=1 if kind == "u16" else 0)
+    assert api.write_tiff(path, px, opts) == ts.Status.success
+    st, back = api.read_tiff(path)
+    assert st == ts.Status.success and np.array_equal(back, px)
+    got = calibrate_on_gpu(back, 100.0, 1.25)
+    want = orc.calibrate(px, 100.0, 1.25)
+    ok, bad = same_or_both_nan(got.reshape(-1), np.asarray(want).view(np.uint32).reshape(-1))
+    assert ok, bad
