@@ -431,20 +431,28 @@ __global__ void __launch_bounds__(Cfg<U16>::threads) __attribute__((amdgpu_waves
             std::int64_t top = static_cast<std::int64_t>(t) - 1;
             std::uint32_t spins = 0;
             for (;;) {
+                // One 128-byte line of look-back words per round trip (a poll's
+                // latency grows with the lines it touches): lane k reads tile
+                // top - k, down to the first word of top's line.
+                const std::uint32_t width =
+                    static_cast<std::uint32_t>((reinterpret_cast<std::uintptr_t>(&state[top]) >> 3) & 15u) + 1u;
+                const bool active = lane < width;
                 const std::int64_t j = top - static_cast<std::int64_t>(lane);
-                const std::uint64_t s = j >= static_cast<std::int64_t>(e.first) ? load_state(&state[j]) : 0ull;
+                const std::uint64_t s =
+                    active && j >= static_cast<std::int64_t>(e.first) ? load_state(&state[j]) : 0ull;
                 const std::uint64_t flag = s & kFlagMask;
-                const std::uint64_t prefix_lanes = __ballot(flag == kFlagPrefix);
-                const std::uint64_t empty_lanes = __ballot(flag == 0ull);
+                const std::uint64_t prefix_lanes = __ballot(active && flag == kFlagPrefix);
+                const std::uint64_t empty_lanes = __ballot(active && flag == 0ull);
                 // The nearest published prefix, and whether every tile before it has published.
-                const std::uint32_t stop = prefix_lanes != 0 ? static_cast<std::uint32_t>(__builtin_ctzll(prefix_lanes)) : 64u;
-                const std::uint64_t needed = stop == 64u ? ~0ull : ((2ull << stop) - 1ull);
+                const std::uint32_t stop =
+                    prefix_lanes != 0 ? static_cast<std::uint32_t>(__builtin_ctzll(prefix_lanes)) : width;
+                const std::uint64_t needed = stop == width ? ((1ull << width) - 1ull) : ((2ull << stop) - 1ull);
                 if ((empty_lanes & needed) == 0) {
-                    const bool mine = lane <= stop;
+                    const bool mine = active && lane <= stop;
                     exclusive += wave_sum(mine ? (s & kValueMask) : 0ull);
                     failed = failed || __ballot(mine && (s & kFailed) != 0) != 0;
-                    if (stop != 64u) break;
-                    top -= 64;
+                    if (stop != width) break;
+                    top -= width;
                     continue;
                 }
                 if (++spins > kSpinLimit) {
