@@ -548,11 +548,14 @@ Status build(const BatchRequest &request, PlanState &plan, hipStream_t stream) {
     plan.tasks = reinterpret_cast<const tsk_decode_task *>(base + tasks_at);
     plan.rows = reinterpret_cast<const std::uint16_t *>(base + rows_at);
     plan.task_count = d.decode.tasks.size();
-    // Plans of at least 2048 tiles (eight per CU) launch 1024 workgroups that
-    // stride over the tiles with the next payload in flight while one decodes;
-    // one workgroup per tile exposes each workgroup's load latency there
-    // (4096-tile images: 13-20% slower).  Smaller plans keep one per tile.
-    if (plan.task_count >= 2048) plan.shape = {4, 256, 1024, 3};
+    // Large plans launch 1024 workgroups that stride over the tiles with the
+    // next payload in flight while one decodes; one workgroup per tile exposes
+    // each workgroup's load latency there.  Measured crossover on single
+    // uint16 images with 64x64 tiles: 1536 tiles for fused calibration (4096
+    // tiles: 47 vs 58 us), beyond 4096 tiles for plain decodes (8192 tiles: 40
+    // vs 46 us; 4096 tiles: 23 vs 21 us).  Smaller plans keep one per tile.
+    const std::uint64_t stride_from = request.operation == Operation::decode_calibrate ? 1536 : 8192;
+    if (plan.task_count >= stride_from) plan.shape = {4, 256, 1024, 3};
     plan.encode_images = reinterpret_cast<const tsk_encode_image *>(base + enc_images_at);
     plan.calibrate_images = reinterpret_cast<const tsk_calibrate_image *>(base + cal_at);
     plan.first_rows = reinterpret_cast<const std::uint64_t *>(base + first_at);
