@@ -188,3 +188,25 @@ def test_tiff_file_to_calibrated_output(orc, tmp_path, kind):
     want = orc.calibrate(px, 100.0, 1.25)
     ok, bad = same_or_both_nan(got.reshape(-1), np.asarray(want).view(np.uint32).reshape(-1))
     assert ok, bad
+
+
+@pytest.mark.parametrize("kind", ["u16", "f32"])
+def test_large_fused_plan_striding_launch(orc, kind):
+    """Fused plans of 1536 tiles or more launch 1024 workgroups that stride over
+    the tiles; their words equal the one-workgroup-per-tile launch's and the
+    oracle's (NaNs by class)."""
+    w, h = 2600, 2400  # 41 x 38 = 1558 tiles of 64x64, ragged last column and row
+    px = g.detector_u16(w, h, 34) if kind == "u16" else g.field_f32(w, h, 34)
+    data = orc.compress(px, (64, 64))
+    dark = g.dark_u16(w, h, 35).astype(np.float32)
+    gain = g.gain_f32(w, h, 36)
+    outs = []
+    for shape in (None, (8, 256, 0, 1)):
+        job = Job()
+        job.add_decode(data, calibrated=True, offset=dark, gain=gain)
+        status, written, o = job.run(ts.Operation.decode_calibrate, shape)
+        assert status[0] == 0 and written[0] == w * h * 4
+        outs.append(np.ascontiguousarray(o[0]).view(np.uint32))
+    assert np.array_equal(outs[0], outs[1])
+    ok, bad = same_or_both_nan(outs[0], orc.decode_roi_calibrate(data, (0, 0, w, h), dark, gain).view(np.uint32))
+    assert ok, bad
