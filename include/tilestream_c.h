@@ -142,8 +142,8 @@ uint32_t ts_synchronize(void *stream);
 /* group: columns per lane (1,2,4,8,16); block: 64..512; grid: 0 = one workgroup per tile;
  * depth: 1 staged; 2 or 3: one or two more tiles in flight.  invalid_argument for a combination
  * without a kernel.  A new plan's shape: group 4, block 256, grid 1024, depth 3 for fused
- * calibration plans of 1536 tiles or more and other decode plans of 8192 or more; group 8,
- * block 256, grid 0, depth 1 otherwise. */
+ * calibration plans of 1536 tiles or more; group 8, block 256, grid 1024, depth 3 for other
+ * decode plans of 2048 or more; group 8, block 256, grid 0, depth 1 otherwise. */
 uint32_t ts_plan_set_decode_shape(ts_plan *plan, uint32_t group, uint32_t block, uint32_t grid, uint32_t depth);
 
 /* ---- extension: batched persistent decode (tilestream/batch.h) ---------

@@ -155,3 +155,19 @@ def test_unselected_payloads_poisoned(orc):
     status, _, outs = job.run(ts.Operation.decode_roi)
     assert status[0] == 0
     assert np.array_equal(outs[0], px[70:120, 70:170])
+
+
+@pytest.mark.parametrize("kind", ["u16", "f32"])
+def test_large_plan_striding_launch(orc, kind):
+    """Decode plans of 2048 tiles or more launch 1024 workgroups that stride over
+    the tiles; the image equals the one-workgroup-per-tile launch's and the
+    source pixels."""
+    w, h = 4100, 2090  # 65 x 33 = 2145 tiles of 64x64, ragged last column and row
+    px = g.detector_u16(w, h, 71) if kind == "u16" else g.field_f32(w, h, 71)
+    data = orc.compress(px, (64, 64))
+    striding, _ = decode_on_gpu(data)
+    per_tile, n = decode_on_gpu(data, shape=(8, 256, 0, 1))
+    assert n == px.nbytes
+    word = np.uint16 if kind == "u16" else np.uint32
+    assert np.array_equal(np.ascontiguousarray(striding).view(word), np.ascontiguousarray(per_tile).view(word))
+    assert np.array_equal(np.ascontiguousarray(striding).view(word), np.ascontiguousarray(px).view(word))

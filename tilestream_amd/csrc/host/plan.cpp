@@ -550,12 +550,16 @@ Status build(const BatchRequest &request, PlanState &plan, hipStream_t stream) {
     plan.task_count = d.decode.tasks.size();
     // Large plans launch 1024 workgroups that stride over the tiles with the
     // next payload in flight while one decodes; one workgroup per tile exposes
-    // each workgroup's load latency there.  Measured crossover on single
-    // uint16 images with 64x64 tiles: 1536 tiles for fused calibration (4096
-    // tiles: 47 vs 58 us), beyond 4096 tiles for plain decodes (8192 tiles: 40
-    // vs 46 us; 4096 tiles: 23 vs 21 us).  Smaller plans keep one per tile.
-    const std::uint64_t stride_from = request.operation == Operation::decode_calibrate ? 1536 : 8192;
-    if (plan.task_count >= stride_from) plan.shape = {4, 256, 1024, 3};
+    // each workgroup's load latency there.  Measured on single uint16 images
+    // with 64x64 tiles: fused calibration from 1536 tiles with 4 columns per
+    // lane (4096 tiles: 47 vs 60 us), plain decodes from 2048 tiles with 8
+    // (2048 / 4096 / 8192 tiles: 13 / 20 / 35 vs 15 / 21 / 47 us).  Smaller
+    // plans keep one workgroup per tile.
+    if (request.operation == Operation::decode_calibrate) {
+        if (plan.task_count >= 1536) plan.shape = {4, 256, 1024, 3};
+    } else if (plan.task_count >= 2048) {
+        plan.shape = {8, 256, 1024, 3};
+    }
     plan.encode_images = reinterpret_cast<const tsk_encode_image *>(base + enc_images_at);
     plan.calibrate_images = reinterpret_cast<const tsk_calibrate_image *>(base + cal_at);
     plan.first_rows = reinterpret_cast<const std::uint64_t *>(base + first_at);
