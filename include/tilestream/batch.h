@@ -25,6 +25,7 @@ struct LaunchShape {
     std::uint32_t grid = 0;    // workgroups; 0 = resident maximum of the device
     std::uint32_t chunk = 8;   // tasks per draw from the work counter (last eighth); 0 = static stride only
     std::uint32_t depth = 2;   // tiles a workgroup holds: 1 = staged only, 2 or 3 = 1 or 2 more in flight
+    std::uint32_t run = 1;     // neighbouring tasks a workgroup takes in a row from the static stride
 };
 
 struct QueueStats {

@@ -151,7 +151,7 @@ uint32_t ts_plan_set_decode_shape(ts_plan *plan, uint32_t group, uint32_t block,
  * regions and fused calibration; operations[i] is 1 decompress, 2 decode_roi
  * or 4 decode_calibrate. */
 typedef struct ts_queue ts_queue;
-typedef struct { uint32_t group, block, grid, chunk, depth; } ts_launch_shape;
+typedef struct { uint32_t group, block, grid, chunk, depth, run; } ts_launch_shape;
 typedef struct {
     uint64_t images, tasks, decoded_bytes, output_bytes, payload_bytes, map_bytes;
 } ts_queue_stats;

@@ -112,6 +112,11 @@ typedef struct {
      * within its 16-byte block. */
     const uint32_t *staged;
     const uint8_t *staging;
+    /* Static stride in runs: each workgroup takes `run` consecutive tasks
+     * (neighbouring tiles, whose outputs form one compact block) before it
+     * strides on; 0 or 1 = single tasks. */
+    uint32_t run;
+    uint32_t reserved;
 } tsk_decode_args;
 
 /* Per-image descriptor of an encode job. */

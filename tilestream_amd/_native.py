@@ -121,7 +121,7 @@ class BatchRequest(C.Structure):
 
 class LaunchShape(C.Structure):
     _fields_ = [("group", C.c_uint32), ("block", C.c_uint32), ("grid", C.c_uint32), ("chunk", C.c_uint32),
-                ("depth", C.c_uint32)]
+                ("depth", C.c_uint32), ("run", C.c_uint32)]
 
 
 class QueueStats(C.Structure):

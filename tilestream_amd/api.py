@@ -528,8 +528,9 @@ class DecodeQueue:
         N.lib().ts_queue_get_shape(self.handle, C.byref(s))
         return s
 
-    def set_shape(self, group=8, block=256, grid=0, chunk=8, depth=2) -> Status:
-        return Status(N.lib().ts_queue_set_shape(self.handle, C.byref(LaunchShape(group, block, grid, chunk, depth))))
+    def set_shape(self, group=8, block=256, grid=0, chunk=8, depth=2, run=1) -> Status:
+        return Status(N.lib().ts_queue_set_shape(self.handle,
+                                                 C.byref(LaunchShape(group, block, grid, chunk, depth, run))))
 
     def _ws(self, workspace):
         return MutableBytes(workspace.ptr, workspace.size) if workspace is not None else MutableBytes(None, 0)

@@ -576,7 +576,10 @@ struct Cursor {
     std::uint64_t next, end;
     std::uint64_t ahead; // the chunk requested one chunk early (thread 0)
     bool dynamic;
+    std::uint32_t left;  // tasks left in the current static run
 };
+
+__device__ inline std::uint32_t run_length(const tsk_decode_args &a) { return a.run > 1 ? a.run : 1u; }
 
 __device__ inline std::uint64_t static_end(const tsk_decode_args &a) { return a.task_count - a.task_count / 8; }
 
@@ -590,6 +593,12 @@ __device__ inline std::uint64_t stride_slot() {
     return g % 8 == 0 ? std::uint64_t{b % 8} * (g / 8) + b / 8 : b;
 }
 
+// The cursor of a workgroup: slot s takes tasks [s run, s run + run), then
+// strides on by grid x run.
+__device__ inline Cursor start_cursor(const tsk_decode_args &a) {
+    return Cursor{stride_slot() * run_length(a), 0, 0, false, run_length(a)};
+}
+
 // Thread 0 requests the first dynamic chunk at kernel start.
 __device__ inline void queue_start(const tsk_decode_args &a, Cursor &c) {
     if (threadIdx.x == 0) c.ahead = atomicAdd(reinterpret_cast<unsigned long long *>(a.queue), a.chunk);
@@ -600,7 +609,12 @@ __device__ inline std::uint64_t draw(const tsk_decode_args &a, Cursor &c, std::u
     if (a.queue == nullptr || !c.dynamic) {
         if (a.queue == nullptr || c.next < static_end(a)) {
             const std::uint64_t i = c.next;
-            c.next += gridDim.x;
+            if (--c.left == 0) {
+                c.left = run_length(a);
+                c.next += std::uint64_t{gridDim.x - 1} * c.left + 1;
+            } else {
+                ++c.next;
+            }
             return i;
         }
         c.dynamic = true;
@@ -715,7 +729,7 @@ __device__ inline __attribute__((always_inline)) void decode_body_staged(const t
     __shared__ std::uint64_t shared_next[1];
 
     const std::uint32_t tid = threadIdx.x;
-    Cursor cursor{stride_slot(), 0, 0, false};
+    Cursor cursor = start_cursor(a);
     if (a.queue != nullptr) queue_start(a, cursor);
     for (;;) {
         const std::uint64_t i = draw<BLOCK>(a, cursor, shared_next);
@@ -911,7 +925,7 @@ __device__ inline __attribute__((always_inline)) void decode_body_prefetch(const
     __shared__ std::uint16_t row_start[2][64];
     __shared__ std::uint32_t shared_flag;
     __shared__ std::uint64_t shared_next[1];
-    Pipeline<G, KIND, BLOCK, DEPTH> pipe{a, stage, row_start, &shared_flag, shared_next, Cursor{stride_slot(), 0, 0, false}};
+    Pipeline<G, KIND, BLOCK, DEPTH> pipe{a, stage, row_start, &shared_flag, shared_next, start_cursor(a)};
     pipe.run();
 }
 

@@ -182,6 +182,7 @@ Status DecodeQueue::run(MutableBytes workspace, ResultBuffer results, hipStream_
     a.grid = grid;
     a.chunk = sh.chunk;
     a.depth = sh.depth;
+    a.run = sh.run;
     if (tsk_decode(&a, stream) != hipSuccess ||
         tsk_results_complete(q.images, static_cast<std::uint32_t>(q.count), results.data, stream) != hipSuccess)
         return detail::hip_failure();

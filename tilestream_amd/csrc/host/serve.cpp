@@ -580,6 +580,7 @@ Status Server::Impl::launch(Slot &slot, const ServeRequest *requests, const std:
         args.grid = static_cast<std::uint32_t>(std::max<std::uint64_t>(1, grid));
         args.chunk = queued ? sh.chunk : 0;
         args.depth = sh.depth;
+        args.run = sh.run;
         args.request_count = static_cast<std::uint32_t>(regions);
         args.requests = reinterpret_cast<const tsk_request *>(slot.device + l.requests);
         args.request_index = reinterpret_cast<const std::uint32_t *>(slot.device + l.index);
