@@ -186,4 +186,16 @@ class BatchPlan {
 [[nodiscard]] Status decode_calibrate_async(const BatchPlan &plan, MutableBytes workspace,
                                             ResultBuffer results, hipStream_t stream) noexcept;
 
+// compress_async without a workspace (no counterpart in the reference): for
+// callers that cannot allocate device memory (code under graph capture, tight
+// serving loops).  The encoder's look-back state lives in a static 1 MiB
+// device array of the library, one per GPU, held until the process ends; the
+// whole batch runs as one launch when its state fits (65,535 tiles), else by
+// windows of at most 4096 tiles, and a plan whose state cannot fit returns
+// resource_limit.  Calls share that array: calls that overlap on one GPU (two
+// streams, two host threads) race and must be ordered by the caller.  It
+// allocates and frees nothing.
+[[nodiscard]] Status compress_async_static(const BatchPlan &plan, ResultBuffer results,
+                                           hipStream_t stream) noexcept;
+
 } // namespace tilestream::hip

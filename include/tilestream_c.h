@@ -125,6 +125,14 @@ void ts_plan_destroy(ts_plan *plan);
 void ts_plan_workspace_spec(const ts_plan *plan, ts_workspace_spec *out);
 uint32_t ts_compress_async(const ts_plan *plan, ts_mutable_bytes workspace, ts_device_result *results,
                            uint64_t result_count, void *stream);
+/* ts_compress_async without a workspace.  No counterpart in the reference:
+ * for callers that cannot allocate device memory (graph capture, serving
+ * loops).  The encoder state lives in a static 1 MiB device array per GPU
+ * (held until the process ends); a plan whose state does not fit returns
+ * resource_limit (8).  Calls that overlap on one GPU share that array and
+ * race.  Allocates and frees nothing. */
+uint32_t ts_compress_async_static(const ts_plan *plan, ts_device_result *results, uint64_t result_count,
+                                  void *stream);
 uint32_t ts_decompress_async(const ts_plan *plan, ts_mutable_bytes workspace, ts_device_result *results,
                              uint64_t result_count, void *stream);
 uint32_t ts_decode_roi_async(const ts_plan *plan, ts_mutable_bytes workspace, ts_device_result *results,

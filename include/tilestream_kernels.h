@@ -202,6 +202,12 @@ int tsk_encode_reset(const tsk_encode_args *args, void *stream);
  * in it the header and result.  Windows of one execution run in order. */
 int tsk_encode_window(const tsk_encode_args *args, void *stream);
 int tsk_calibrate(const tsk_calibrate_args *args, void *stream);
+/* Size of the encoder's static device scratch (look-back buffers and carried
+ * cursors of the allocation-free encode entry point): 1 MiB per GPU. */
+#define TSK_SCRATCH_BYTES (1u << 20)
+/* Device address and size of that scratch on the current device, looked up
+ * on the first call for each device and kept until the process ends. */
+int tsk_scratch_region(void **address, uint64_t *bytes);
 /* Loads every kernel of an operation so that no execution loads code.
  * op: 0 compress, 1 decode, 2 calibrate, 3 fused decode. */
 int tsk_preload(uint32_t op);

@@ -201,6 +201,7 @@ PROTOTYPES = {
     "ts_plan_destroy": (None, [_vp]),
     "ts_plan_workspace_spec": (None, [_vp, _P(WorkspaceSpec)]),
     "ts_compress_async": (_u32, [_vp, MutableBytes, _vp, _u64, _vp]),
+    "ts_compress_async_static": (_u32, [_vp, _vp, _u64, _vp]),
     "ts_decompress_async": (_u32, [_vp, MutableBytes, _vp, _u64, _vp]),
     "ts_decode_roi_async": (_u32, [_vp, MutableBytes, _vp, _u64, _vp]),
     "ts_calibrate_async": (_u32, [_vp, MutableBytes, _vp, _u64, _vp]),

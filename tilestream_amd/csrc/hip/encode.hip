@@ -32,7 +32,9 @@
 
 #include <tilestream_kernels.h>
 
+#include <atomic>
 #include <cstdint>
+#include <mutex>
 #include <type_traits>
 
 namespace {
@@ -558,6 +560,20 @@ __global__ void __launch_bounds__(Cfg<U16>::threads) __attribute__((amdgpu_waves
     }
 }
 
+// Device memory of the allocation-free encode entry point: both look-back
+// buffers and the two carried cursors, held by every GPU the library runs on
+// until the process ends.  Host code never passes this name to a kernel (on
+// the host it is only a stand-in for the symbol); it uses the per-device
+// address returned by tsk_scratch_region.
+constexpr std::uint64_t kScratchBytes = std::uint64_t{TSK_SCRATCH_BYTES};
+static __device__ __align__(256) std::uint8_t scratch[kScratchBytes];
+
+// Device address of `scratch` on each device, looked up once per device with
+// that device current.
+constexpr int kMaxDevices = 64;
+std::atomic<void *> scratch_address[kMaxDevices];
+std::mutex scratch_lookup;
+
 __global__ void clear_kernel(std::uint64_t *state, std::uint64_t words) {
     const std::uint64_t stride = std::uint64_t{gridDim.x} * blockDim.x;
     for (std::uint64_t k = std::uint64_t{blockIdx.x} * blockDim.x + threadIdx.x; k < words; k += stride) state[k] = 0;
@@ -583,6 +599,26 @@ int tsk_encode_window(const tsk_encode_args *a, void *stream) {
         hipLaunchKernelGGL(encode_kernel<false>, dim3(a->tile_count), dim3(Cfg<false>::threads), 0,
                            static_cast<hipStream_t>(stream), *a);
     return hipGetLastError();
+}
+
+int tsk_scratch_region(void **address, std::uint64_t *bytes) {
+    int device = 0;
+    const hipError_t e = hipGetDevice(&device);
+    if (e != hipSuccess) return e;
+    if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+    void *p = scratch_address[device].load(std::memory_order_acquire);
+    if (p == nullptr) {
+        const std::lock_guard<std::mutex> hold(scratch_lookup);
+        p = scratch_address[device].load(std::memory_order_relaxed);
+        if (p == nullptr) {
+            const hipError_t found = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+            if (found != hipSuccess || p == nullptr) return found != hipSuccess ? found : hipErrorInvalidSymbol;
+            scratch_address[device].store(p, std::memory_order_release);
+        }
+    }
+    *address = p;
+    *bytes = kScratchBytes;
+    return hipSuccess;
 }
 
 int tsk_preload_encode(void) {

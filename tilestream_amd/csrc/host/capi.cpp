@@ -248,6 +248,10 @@ void ts_plan_workspace_spec(const ts_plan *plan, ts_workspace_spec *out) {
 uint32_t ts_compress_async(const ts_plan *p, ts_mutable_bytes ws, ts_device_result *r, uint64_t n, void *s) {
     return exec(hip::compress_async, p, ws, r, n, s);
 }
+uint32_t ts_compress_async_static(const ts_plan *p, ts_device_result *r, uint64_t n, void *s) {
+    if (p == nullptr || p->plan == nullptr) return code(Status::invalid_argument);
+    return code(hip::compress_async_static(*p->plan, {reinterpret_cast<DeviceResult *>(r), n}, stream_of(s)));
+}
 uint32_t ts_decompress_async(const ts_plan *p, ts_mutable_bytes ws, ts_device_result *r, uint64_t n, void *s) {
     return exec(hip::decompress_async, p, ws, r, n, s);
 }

@@ -700,35 +700,34 @@ Status decode_calibrate_async(const BatchPlan &plan, MutableBytes workspace, Res
     return decode_entry(plan, Operation::decode_calibrate, workspace, results, stream);
 }
 
-Status compress_async(const BatchPlan &plan, MutableBytes workspace, ResultBuffer results, hipStream_t stream) noexcept {
-    const detail::PlanState *state = plan.impl();
-    const Status st = detail::check_execution(state, Operation::compress, workspace, results, stream);
-    if (st != Status::success || state->count == 0) return st;
-    auto *ws = static_cast<std::uint8_t *>(workspace.data);
+namespace {
+
+// Enqueues the encoder over a plan's tiles: one launch over the whole batch
+// when `whole`, else one per window.  `state` holds the two look-back buffers
+// (2 x capacity words) and `carry` the two carried cursors.
+Status enqueue_encode(const detail::PlanState &plan, std::uint64_t *state, std::uint64_t *carry, bool whole,
+                      DeviceResult *results, hipStream_t stream) noexcept {
     tsk_encode_args a{};
-    a.images = state->encode_images;
-    a.results = results.data;
-    // Look-back buffers in the residual region (16 KiB per window tile), the
-    // carried cursor in the image region.
-    a.state = reinterpret_cast<std::uint64_t *>(ws + state->layout.residuals);
-    a.carry = reinterpret_cast<std::uint64_t *>(ws + state->layout.images);
-    a.capacity = static_cast<std::uint32_t>(state->spec.tile_count);
-    a.type = static_cast<std::uint32_t>(state->type);
-    if (state->whole.tile_count != 0) {
-        a.capacity = state->whole.tile_count;
-        a.entries = state->whole.entries;
-        a.blocks = state->whole.blocks;
-        a.entry_count = state->whole.entry_count;
-        a.tile_count = state->whole.tile_count;
+    a.images = plan.encode_images;
+    a.results = results;
+    a.state = state;
+    a.carry = carry;
+    a.type = static_cast<std::uint32_t>(plan.type);
+    if (whole) {
+        a.capacity = plan.whole.tile_count;
+        a.entries = plan.whole.entries;
+        a.blocks = plan.whole.blocks;
+        a.entry_count = plan.whole.entry_count;
+        a.tile_count = plan.whole.tile_count;
         a.parity = 0;
         if (tsk_encode_reset(&a, stream) != hipSuccess || tsk_encode_window(&a, stream) != hipSuccess)
             return detail::hip_failure();
         return Status::success;
     }
-    if (tsk_encode_reset(&a, stream) != hipSuccess)
-        return detail::hip_failure();
-    for (std::size_t k = 0; k < state->windows.size(); ++k) {
-        const detail::EncodeWindow &w = state->windows[k];
+    a.capacity = static_cast<std::uint32_t>(plan.spec.tile_count);
+    if (tsk_encode_reset(&a, stream) != hipSuccess) return detail::hip_failure();
+    for (std::size_t k = 0; k < plan.windows.size(); ++k) {
+        const detail::EncodeWindow &w = plan.windows[k];
         a.entries = w.entries;
         a.blocks = w.blocks;
         a.entry_count = w.entry_count;
@@ -737,6 +736,51 @@ Status compress_async(const BatchPlan &plan, MutableBytes workspace, ResultBuffe
         if (tsk_encode_window(&a, stream) != hipSuccess) return detail::hip_failure();
     }
     return Status::success;
+}
+
+} // namespace
+
+Status compress_async(const BatchPlan &plan, MutableBytes workspace, ResultBuffer results, hipStream_t stream) noexcept {
+    const detail::PlanState *state = plan.impl();
+    const Status st = detail::check_execution(state, Operation::compress, workspace, results, stream);
+    if (st != Status::success || state->count == 0) return st;
+    auto *ws = static_cast<std::uint8_t *>(workspace.data);
+    // Look-back buffers in the residual region (16 KiB per window tile), the
+    // carried cursor in the image region.
+    return enqueue_encode(*state, reinterpret_cast<std::uint64_t *>(ws + state->layout.residuals),
+                          reinterpret_cast<std::uint64_t *>(ws + state->layout.images), state->whole.tile_count != 0,
+                          results.data, stream);
+}
+
+Status compress_async_static(const BatchPlan &plan, ResultBuffer results, hipStream_t stream) noexcept {
+    const detail::PlanState *state = plan.impl();
+    // The checks of compress_async for everything but the workspace.
+    if (state == nullptr || state->operation != Operation::compress) return Status::invalid_argument;
+    if (state->count == 0) return Status::success;
+    std::uint64_t result_bytes = 0;
+    if (!host::mul(results.count, sizeof(DeviceResult), result_bytes)) return Status::size_overflow;
+    if (const Status st = host::byte_range(results.data, result_bytes); st != Status::success) return st;
+    if (results.count < state->count) return Status::buffer_too_small;
+    if (reinterpret_cast<std::uintptr_t>(results.data) % alignof(DeviceResult) != 0) return Status::invalid_argument;
+    const auto at = reinterpret_cast<std::uintptr_t>(results.data);
+    if (!state->ranges.disjoint({at, at + static_cast<std::uintptr_t>(result_bytes)})) return Status::invalid_argument;
+    Status st = detail::stream_device(stream, state->device);
+    if (st == Status::success) st = detail::accessible(results.data, state->device, state->pageable);
+    if (st == Status::success && hipPeekAtLastError() != hipSuccess) st = Status::hip_error;
+    if (st != Status::success) return st;
+
+    void *scratch = nullptr;
+    std::uint64_t scratch_bytes = 0;
+    if (tsk_scratch_region(&scratch, &scratch_bytes) != hipSuccess) return detail::hip_failure();
+    // Look-back buffers first (2 x 8 bytes per tile of the largest launch),
+    // then the two carried cursors.  The whole batch runs as one launch when
+    // its buffers fit; otherwise by windows of at most 4096 tiles.
+    const auto need = [](std::uint64_t capacity) { return 2 * 8 * capacity + 2 * 8; };
+    const bool whole = state->whole.tile_count != 0 && need(state->whole.tile_count) <= scratch_bytes;
+    const std::uint64_t capacity = whole ? state->whole.tile_count : state->spec.tile_count;
+    if (need(capacity) > scratch_bytes) return Status::resource_limit;
+    auto *words = static_cast<std::uint64_t *>(scratch);
+    return enqueue_encode(*state, words, words + 2 * capacity, whole, results.data, stream);
 }
 
 Status calibrate_async(const BatchPlan &plan, MutableBytes workspace, ResultBuffer results, hipStream_t stream) noexcept {
