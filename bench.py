@@ -1208,6 +1208,8 @@ def main() -> int:
                                                                     (512, 2), (512, 3))
                   for c in (4, 16)]
     candidates += [(8, 256, 0, 0, 2), (4, 256, 0, 0, 3)]
+    # The block decoder (group 0): 512 threads, two workgroups per CU.
+    candidates += [(0, 512, 0, c, 2) for c in (1, 2, 4)]
     if args.profile:
         grp, blk, chk, dep = (int(v) for v in args.shape.split(","))
         assert queue.set_shape(grp, blk, 0, chk, dep) == api.Status.success

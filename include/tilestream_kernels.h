@@ -88,6 +88,44 @@ typedef struct {
     uint32_t reserved;
 } tsk_request;
 
+/* Block decode: a block is up to 8 tiles of one image, `across` x `down`
+ * neighbouring tiles of its region whose output is at most 64 rows of at
+ * most 1 KiB: a workgroup stages all their payloads in LDS at once and every
+ * wave writes whole 1 KiB runs of one output row. */
+typedef struct {
+    const uint8_t *payload;      /* absolute device address of the tile payload */
+    uint32_t words;              /* payload length / 4 */
+    uint32_t shape;              /* mode | actual height << 8 | actual width << 16 */
+    uint32_t rows;               /* first entry of the row table, TSK_NO_ROWS if none */
+    uint32_t stage;              /* first 16-byte LDS block of the payload's 16-byte blocks */
+    uint32_t reserved[2];
+} tsk_block_tile;
+
+/* 416 bytes (26 16-byte blocks, read by one wave straight into LDS). */
+typedef struct {
+    uint32_t image;
+    uint16_t across, down;       /* tiles of this block */
+    uint32_t col0, row0;         /* archive tile of its top-left tile */
+    uint32_t count;              /* across x down */
+    uint32_t blocks;             /* 16-byte LDS blocks staged for all its tiles */
+    uint32_t flags;              /* TSK_BLOCK_FULL */
+    uint32_t reserved;
+    tsk_block_tile tiles[8];     /* row-major; entries past count are zero */
+    tsk_decode_image image_copy; /* the image record, so that a block needs one read */
+    uint64_t pad;
+} tsk_block;
+
+/* 16-byte LDS blocks of a block's stage: eight RAW 64x64 uint16 tiles, each
+ * 16-byte misaligned, fill 4104; planning rejects more than 4135, so the
+ * farthest read past a row's end (258 words) and the rounding of the last
+ * wave's loads to 64 blocks stay inside the stage. */
+#define TSK_BLOCK_STAGE 4200u
+#define TSK_BLOCK_STAGE_USED 4135u
+
+/* Every tile of the block is predictive and full size, and the block has
+ * its full number of tiles both ways. */
+#define TSK_BLOCK_FULL 1u
+
 typedef struct {
     const tsk_decode_image *images;
     const tsk_decode_task *tasks;
@@ -98,7 +136,7 @@ typedef struct {
     uint64_t row_count;          /* entries in the row table */
     uint32_t image_count;
     uint32_t kind;               /* TSK_KIND_* */
-    uint32_t group;              /* columns per lane: 1, 2, 4, 8 or 16 */
+    uint32_t group;              /* columns per lane: 1, 2, 4, 8 or 16; 0: block decode */
     uint32_t block;              /* threads per workgroup: 64, 128, 256 or 512 */
     uint32_t grid;               /* workgroups; tiles are strided or drawn from the queue */
     uint32_t chunk;              /* tiles per queue draw (the last eighth of the tasks; the rest by stride) */
@@ -117,6 +155,8 @@ typedef struct {
      * strides on; 0 or 1 = single tasks. */
     uint32_t run;
     uint32_t reserved;
+    /* Block decode (group 0): task_count counts blocks. */
+    const tsk_block *blocks;
 } tsk_decode_args;
 
 /* Per-image descriptor of an encode job. */

@@ -17,6 +17,16 @@ def batch():
     return ResidentBatch.build(layout, 77)
 
 
+@pytest.fixture(scope="module")
+def aligned_batch():
+    # Output rows on 256-byte boundaries: the block decoder's full-block path
+    # and its non-temporal stores.
+    layout = g.batch_layout(16, 91, lo=256, hi=1600, step=16)
+    return ResidentBatch.build(layout, 91, pitch_align=256)
+
+BLOCK_SHAPES = [(0, 512, 0, 1, 2), (0, 512, 0, 0, 2), (0, 512, 7, 2, 2), (0, 512, 0, 16, 2)]
+
+
 def _run_queue(batch, ops, rois, shape=None, stream=None, graph_replays=0, cal=None):
     items = []
     for i, (op, roi) in enumerate(zip(ops, rois)):
@@ -52,7 +62,7 @@ def test_whole_batch_matches_pixels(batch):
     n = len(batch.entries)
     for shape in [(8, 256, 0, 1, 2), (4, 128, 0, 4, 1), (16, 512, 0, 2, 2), (8, 256, 0, 0, 2),
                   (2, 256, 37, 1, 1), (4, 512, 5, 3, 2), (8, 64, 0, 16, 1), (4, 256, 0, 4, 3),
-                  (8, 512, 7, 2, 3), (16, 256, 0, 0, 3), (4, 128, 0, 4, 2)]:
+                  (8, 512, 7, 2, 3), (16, 256, 0, 0, 3), (4, 128, 0, 4, 2)] + BLOCK_SHAPES:
         batch.clear_outputs()
         ops = [ts.Operation.decompress] * n
         rois = [(0, 0, e.width, e.height) for e in batch.entries]
@@ -61,6 +71,20 @@ def test_whole_batch_matches_pixels(batch):
         for i, e in enumerate(batch.entries):
             assert written[i] == e.raw_bytes
             assert np.array_equal(batch.read_output(i), e.pixels), (shape, i)
+
+
+@pytest.mark.parametrize("shape", BLOCK_SHAPES + [(4, 256, 0, 4, 3)])
+def test_block_decoder_on_aligned_rows(aligned_batch, shape):
+    batch = aligned_batch
+    n = len(batch.entries)
+    batch.clear_outputs()
+    ops = [ts.Operation.decompress] * n
+    rois = [(0, 0, e.width, e.height) for e in batch.entries]
+    (status, written), = list(_run_queue(batch, ops, rois, shape))
+    assert (status == 0).all()
+    for i, e in enumerate(batch.entries):
+        assert written[i] == e.raw_bytes
+        assert np.array_equal(batch.read_output(i), e.pixels), (shape, i)
 
 
 def test_graph_replays_are_identical(batch):
@@ -73,7 +97,8 @@ def test_graph_replays_are_identical(batch):
             assert np.array_equal(batch.read_output(i), e.pixels)
 
 
-def test_mixed_regions_and_calibration_alone_and_in_mix(batch, orc):
+@pytest.mark.parametrize("shape", [None, (0, 512, 0, 2, 2)])
+def test_mixed_regions_and_calibration_alone_and_in_mix(batch, orc, shape):
     rng = np.random.default_rng(78)
     n = len(batch.entries)
     ops, rois, cal = [], [], {}
@@ -89,7 +114,7 @@ def test_mixed_regions_and_calibration_alone_and_in_mix(batch, orc):
         ops.append(op)
     # in the mix
     batch.clear_outputs()
-    (status, _), = list(_run_queue(batch, ops, rois, cal=cal))
+    (status, _), = list(_run_queue(batch, ops, rois, shape, cal=cal))
     assert (status == 0).all()
     mixed = {}
     for i, e in enumerate(batch.entries):
@@ -109,7 +134,7 @@ def test_mixed_regions_and_calibration_alone_and_in_mix(batch, orc):
         batch.clear_outputs()
         x, y, w, h = rois[i]
         sub = ResidentBatch([batch.entries[i]], batch.src, batch.out)
-        (status, _), = list(_run_queue(sub, [ops[i]], [rois[i]], cal={0: cal[i]} if i in cal else None))
+        (status, _), = list(_run_queue(sub, [ops[i]], [rois[i]], shape, cal={0: cal[i]} if i in cal else None))
         assert status[0] == 0
         e = batch.entries[i]
         if ops[i] == ts.Operation.decode_calibrate:

@@ -12,6 +12,7 @@ DecodeFn tsk_decode_table_f32(std::uint32_t group, std::uint32_t block, std::uin
 DecodeFn tsk_decode_table_u16_cal(std::uint32_t group, std::uint32_t block, std::uint32_t depth);
 DecodeFn tsk_decode_table_f32_cal(std::uint32_t group, std::uint32_t block, std::uint32_t depth);
 DecodeFn tsk_decode_table_mixed(std::uint32_t group, std::uint32_t block, std::uint32_t depth);
+DecodeFn tsk_block_table(std::uint32_t kind, std::uint32_t block);
 }
 
 namespace {
@@ -105,6 +106,7 @@ __global__ void fill_kernel(std::uint32_t *data, std::uint64_t words, std::uint3
 }
 
 DecodeFn decode_function(std::uint32_t kind, std::uint32_t group, std::uint32_t block, std::uint32_t depth) {
+    if (group == 0) return depth == 2 ? tsk_block_table(kind, block) : nullptr;
     if (!tsk::compiled(group, block, depth)) return nullptr;
     switch (kind) {
     case TSK_KIND_U16: return tsk_decode_table_u16(group, block, depth);
@@ -160,6 +162,7 @@ int tsk_results_publish(const tsk_decode_image *images, std::uint32_t count, con
 }
 
 int tsk_decode_supported(std::uint32_t group, std::uint32_t block, std::uint32_t depth) {
+    if (group == 0) return depth == 2 && block == 512 ? 1 : 0;
     return tsk::compiled(group, block, depth) ? 1 : 0;
 }
 
@@ -167,6 +170,8 @@ int tsk_decode(const tsk_decode_args *args, void *stream) {
     if (args->task_count == 0) return hipSuccess;
     const DecodeFn fn = decode_function(args->kind, args->group, args->block, args->depth);
     if (fn == nullptr || args->grid == 0 || (args->queue != nullptr && args->chunk == 0)) return hipErrorInvalidValue;
+    if (args->group == 0 && (args->blocks == nullptr || args->request_count != 0))
+        return hipErrorInvalidValue;
     hipLaunchKernelGGL(fn, dim3(args->grid), dim3(args->block), 0, static_cast<hipStream_t>(stream), *args);
     return hipGetLastError();
 }
@@ -211,7 +216,7 @@ int tsk_preload_decode(std::uint32_t calibrated) {
                                     calibrated ? std::uint32_t{TSK_KIND_F32_CAL} : std::uint32_t{TSK_KIND_F32},
                                     TSK_KIND_MIXED};
     for (std::uint32_t kind : kinds)
-        for (std::uint32_t g : {1u, 2u, 4u, 8u, 16u})
+        for (std::uint32_t g : {0u, 1u, 2u, 4u, 8u, 16u})
             for (std::uint32_t b : {64u, 128u, 256u, 512u})
                 for (std::uint32_t d : {1u, 2u, 3u}) {
                     const DecodeFn fn = decode_function(kind, g, b, d);

@@ -326,6 +326,59 @@ Status draft_decode_item(const BatchItem &item, Operation operation, bool requir
     return Status::success;
 }
 
+bool plan_blocks(const DecodeDraft &d, std::vector<tsk_block> &blocks) {
+    std::uint64_t task = 0;
+    for (std::size_t i = 0; i < d.images.size(); ++i) {
+        const tsk_decode_image &img = d.images[i];
+        const std::uint64_t count = d.tiles_per_image[i];
+        if (count == 0) continue;
+        // The region's tile grid, row-major in the draft.
+        const std::uint32_t left = img.roi_x / img.tile_w, top = img.roi_y / img.tile_h;
+        const std::uint32_t right = static_cast<std::uint32_t>((std::uint64_t{img.roi_x} + img.roi_w - 1) / img.tile_w);
+        const std::uint32_t cols = right - left + 1;
+        const std::uint64_t rows = count / cols;
+        const std::uint32_t out_sample = img.calibrated ? 4u : (img.type == TSK_UINT16 ? 2u : 4u);
+        const std::uint32_t kw = std::max(1u, 1024u / (std::uint32_t{img.tile_w} * out_sample));
+        const std::uint32_t kh = std::max(1u, 64u / std::uint32_t{img.tile_h});
+        if (kw * kh > 8) return false; // the allowed geometries give at most 8 tiles
+        for (std::uint64_t br = 0; br < rows; br += kh) {
+            for (std::uint32_t bc = 0; bc < cols; bc += kw) {
+                tsk_block b{};
+                b.image = static_cast<std::uint32_t>(i);
+                b.across = static_cast<std::uint16_t>(std::min(kw, cols - bc));
+                b.down = static_cast<std::uint16_t>(std::min<std::uint64_t>(kh, rows - br));
+                b.col0 = left + bc;
+                b.row0 = static_cast<std::uint32_t>(top + br);
+                b.count = std::uint32_t{b.across} * b.down;
+                std::uint32_t staged = 0;
+                bool full = b.across == kw && b.down == kh;
+                for (std::uint32_t kr = 0; kr < b.down; ++kr) {
+                    for (std::uint32_t kc = 0; kc < b.across; ++kc) {
+                        const tsk_decode_task &t = d.tasks[task + (br + kr) * cols + bc + kc];
+                        if (t.payload == nullptr) return false;
+                        const auto p = reinterpret_cast<std::uintptr_t>(t.payload);
+                        tsk_block_tile &bt = b.tiles[kr * b.across + kc];
+                        bt.payload = t.payload;
+                        bt.words = t.words;
+                        bt.shape = t.shape;
+                        bt.rows = t.rows;
+                        bt.stage = staged;
+                        staged += static_cast<std::uint32_t>(((p + std::uint64_t{t.words} * 4 + 15) >> 4) - (p >> 4));
+                        full = full && t.shape == TSK_TASK_SHAPE(2u, img.tile_h, img.tile_w) && t.rows != TSK_NO_ROWS;
+                    }
+                }
+                if (staged > TSK_BLOCK_STAGE_USED) return false;
+                b.blocks = staged;
+                b.flags = full ? TSK_BLOCK_FULL : 0u;
+                b.image_copy = img;
+                blocks.push_back(b);
+            }
+        }
+        task += count;
+    }
+    return true;
+}
+
 namespace {
 
 struct Draft {

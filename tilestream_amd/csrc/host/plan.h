@@ -136,6 +136,13 @@ struct DecodeDraft {
 [[nodiscard]] Status draft_decode_item(const BatchItem &item, Operation operation, bool require_one_type,
                                        DecodeDraft &draft);
 
+// Groups the drafted tiles of every item into blocks for the block decoder:
+// per image, `across` x `down` neighbouring tiles of its region covering at
+// most 64 output rows of at most 1 KiB, with each tile's place in the
+// block's LDS stage.  False when some tile's payload is only known on the
+// device (encoder output decoded without host metadata).
+[[nodiscard]] bool plan_blocks(const DecodeDraft &draft, std::vector<tsk_block> &blocks);
+
 } // namespace tilestream::hip::detail
 
 namespace tilestream::hip {

@@ -23,6 +23,9 @@ struct DecodeQueue::Impl {
     const std::uint16_t *rows = nullptr;
     std::uint64_t task_count = 0;
     std::uint64_t row_count = 0;
+    // Block decoder tables (block_count 0: every shape uses the tile decoder).
+    const tsk_block *blocks = nullptr;
+    std::uint64_t block_count = 0;
     QueueStats stats;
     LaunchShape shape;
     hipGraphExec_t graph = nullptr;
@@ -101,10 +104,14 @@ Status DecodeQueue::prepare(const BatchItem *items, const Operation *operations,
         for (const void *p : d.pointers)
             if (const Status st = detail::accessible(p, impl->device, impl->pageable); st != Status::success) return st;
 
+        std::vector<tsk_block> blocks;
+        if (!detail::plan_blocks(d, blocks)) blocks.clear();
+        const auto next = [](std::uint64_t at, std::uint64_t bytes) { return (at + bytes + 15) / 16 * 16; };
         const std::uint64_t images_at = 0;
-        const std::uint64_t tasks_at = (d.images.size() * sizeof(tsk_decode_image) + 15) / 16 * 16;
-        const std::uint64_t rows_at = tasks_at + (d.tasks.size() * sizeof(tsk_decode_task) + 15) / 16 * 16;
-        const std::uint64_t total = rows_at + d.rows.size() * sizeof(std::uint16_t) + 16;
+        const std::uint64_t tasks_at = next(images_at, d.images.size() * sizeof(tsk_decode_image));
+        const std::uint64_t rows_at = next(tasks_at, d.tasks.size() * sizeof(tsk_decode_task));
+        const std::uint64_t blocks_at = next(rows_at, d.rows.size() * sizeof(std::uint16_t));
+        const std::uint64_t total = next(blocks_at, blocks.size() * sizeof(tsk_block)) + 16;
         if (hipMallocAsync(&impl->storage, total, stream) != hipSuccess) {
             impl->storage = nullptr;
             return detail::hip_failure();
@@ -115,9 +122,12 @@ Status DecodeQueue::prepare(const BatchItem *items, const Operation *operations,
         impl->rows = reinterpret_cast<const std::uint16_t *>(base + rows_at);
         impl->task_count = d.tasks.size();
         impl->row_count = d.rows.size();
+        impl->blocks = reinterpret_cast<const tsk_block *>(base + blocks_at);
+        impl->block_count = blocks.size();
         if (tsk_preload(1) != hipSuccess || tsk_preload(3) != hipSuccess) return detail::hip_failure();
         if (!upload(impl->storage, images_at, d.images, stream) || !upload(impl->storage, tasks_at, d.tasks, stream) ||
-            !upload(impl->storage, rows_at, d.rows, stream) || hipStreamSynchronize(stream) != hipSuccess)
+            !upload(impl->storage, rows_at, d.rows, stream) || !upload(impl->storage, blocks_at, blocks, stream) ||
+            hipStreamSynchronize(stream) != hipSuccess)
             return detail::hip_failure();
         queue.reset(new DecodeQueue(std::move(impl)));
         return Status::success;
@@ -153,6 +163,10 @@ Status DecodeQueue::run(MutableBytes workspace, ResultBuffer results, hipStream_
         host::overlaps(workspace.data, workspace.size, results.data, result_bytes))
         return Status::invalid_argument;
     const LaunchShape &sh = q.shape;
+    // Group 0 selects the block decoder, which needs every payload address on
+    // the host (not the case for encoder output decoded from its device index).
+    const bool block_mode = sh.group == 0;
+    if (block_mode && q.block_count == 0) return Status::invalid_argument;
     std::uint32_t grid = sh.grid;
     if (grid == 0) {
         int cus = 0, per_cu = 0;
@@ -173,7 +187,10 @@ Status DecodeQueue::run(MutableBytes workspace, ResultBuffer results, hipStream_
     a.rows = q.rows;
     a.results = results.data;
     a.queue = sh.chunk != 0 ? counter : nullptr;
-    a.task_count = q.task_count;
+    a.task_count = block_mode ? q.block_count : q.task_count;
+    if (block_mode) {
+        a.blocks = q.blocks;
+    }
     a.row_count = q.row_count;
     a.image_count = static_cast<std::uint32_t>(q.count);
     a.kind = q.kind();
