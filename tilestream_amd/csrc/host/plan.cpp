@@ -20,87 +20,143 @@ namespace detail {
 
 using host::byte_range;
 
-bool RangeSet::assign(std::vector<Range> writable, std::vector<Range> readable) {
-    const auto empty = [](const Range &r) { return r.begin == r.end; };
-    const auto by_begin = [](const Range &a, const Range &b) { return a.begin < b.begin; };
-    writable.erase(std::remove_if(writable.begin(), writable.end(), empty), writable.end());
-    readable.erase(std::remove_if(readable.begin(), readable.end(), empty), readable.end());
-    std::sort(writable.begin(), writable.end(), by_begin);
-    std::sort(readable.begin(), readable.end(), by_begin);
-    for (std::size_t i = 1; i < writable.size(); ++i)
-        if (writable[i].begin < writable[i - 1].end) return false;
-    std::vector<std::uintptr_t> reach(readable.size());
-    std::uintptr_t far = 0;
-    for (std::size_t i = 0; i < readable.size(); ++i) reach[i] = far = std::max(far, readable[i].end);
-    writable_.clear();
-    readable_ = std::move(readable);
-    reach_ = std::move(reach);
-    for (const Range &r : writable)
-        if (!disjoint(r)) return false;
-    writable_ = std::move(writable);
+bool Footprint::admit(std::vector<Span> written, std::vector<Span> read) {
+    // One sweep in order of start address.  A buffer overlaps an earlier one
+    // exactly when it starts before that one ends, so it is enough to track
+    // how far the written buffers seen so far reach, and how far all of them
+    // reach: a written buffer may start at neither, a read one not before the
+    // written reach.
+    struct Tagged {
+        Span span;
+        bool written;
+    };
+    std::vector<Tagged> all;
+    all.reserve(written.size() + read.size());
+    for (const Span &s : written)
+        if (!s.empty()) all.push_back({s, true});
+    for (const Span &s : read)
+        if (!s.empty()) all.push_back({s, false});
+    std::sort(all.begin(), all.end(), [](const Tagged &a, const Tagged &b) { return a.span.lo < b.span.lo; });
+    std::uintptr_t written_reach = 0, any_reach = 0;
+    std::vector<Span> merged;
+    for (const Tagged &t : all) {
+        if (t.span.lo < (t.written ? any_reach : written_reach)) return false;
+        any_reach = std::max(any_reach, t.span.hi);
+        if (t.written) written_reach = std::max(written_reach, t.span.hi);
+        if (!merged.empty() && t.span.lo <= merged.back().hi) merged.back().hi = std::max(merged.back().hi, t.span.hi);
+        else merged.push_back(t.span);
+    }
+    union_ = std::move(merged);
     return true;
 }
 
-bool RangeSet::disjoint(Range r) const noexcept {
-    if (r.begin == r.end) return true;
-    const auto starts_before = [](const Range &item, std::uintptr_t end) { return item.begin < end; };
-    // The last writable range starting before r.end is the only one that can reach into r.
-    const auto w = std::lower_bound(writable_.begin(), writable_.end(), r.end, starts_before);
-    if (w != writable_.begin() && std::prev(w)->end > r.begin) return false;
-    const auto rd = std::lower_bound(readable_.begin(), readable_.end(), r.end, starts_before);
-    const auto n = static_cast<std::size_t>(rd - readable_.begin());
-    return n == 0 || reach_[n - 1] <= r.begin;
+bool Footprint::avoids(Span s) const noexcept {
+    if (s.empty()) return true;
+    // The first piece of the union that ends after s starts is the only one
+    // that can overlap it.
+    const auto piece = std::partition_point(union_.begin(), union_.end(), [&](const Span &u) { return u.hi <= s.lo; });
+    return piece == union_.end() || piece->lo >= s.hi;
+}
+
+WindowSplit split_into_windows(const std::vector<std::uint64_t> &tiles, std::uint32_t window) {
+    WindowSplit out;
+    std::uint64_t total = 0;
+    for (std::uint64_t n : tiles) total += n;
+    std::size_t image = 0;        // first image that may still have tiles at or after w0
+    std::uint64_t image_begin = 0; // its first tile in the concatenation
+    for (std::uint64_t w0 = 0; w0 < total; w0 += window) {
+        const std::uint64_t w1 = std::min<std::uint64_t>(total, w0 + window);
+        WindowSpan span;
+        span.first_entry = out.entries.size();
+        span.tile_count = static_cast<std::uint32_t>(w1 - w0);
+        while (image < tiles.size() && image_begin + tiles[image] <= w0) image_begin += tiles[image++];
+        std::uint64_t begin = image_begin;
+        for (std::size_t i = image; i < tiles.size() && begin < w1; begin += tiles[i++]) {
+            const std::uint64_t end = begin + tiles[i];
+            const std::uint64_t lo = std::max(begin, w0), hi = std::min(end, w1);
+            if (lo >= hi) continue; // an image with no tiles
+            const std::uint32_t flags = (lo > begin ? 1u : 0u) | (hi == end ? 2u : 0u);
+            out.entries.push_back({static_cast<std::uint32_t>(i), static_cast<std::uint32_t>(lo - w0),
+                                   static_cast<std::uint32_t>(hi - lo), flags, lo - begin});
+            ++span.entry_count;
+        }
+        out.windows.push_back(span);
+    }
+    return out;
 }
 
 PlanState::~PlanState() {
-    // Every execution of the plan has completed, so the free needs no ordering.
-    if (storage != nullptr) static_cast<void>(hip_status(hipFreeAsync(storage, hipStreamPerThread)));
-}
-
-// Device memory of `device`, managed memory, mapped pinned host memory, or
-// pageable memory where the device can access it (reference plan.cpp:70-85).
-Status accessible(const void *data, int device, bool pageable) noexcept {
-    hipPointerAttribute_t attr{};
-    const hipError_t e = hipPointerGetAttributes(&attr, data);
-    if (e == hipErrorInvalidValue) {
-        static_cast<void>(hipGetLastError()); // unknown to the runtime: ordinary pageable memory
-        return pageable ? Status::success : Status::invalid_argument;
-    }
-    if (e != hipSuccess) return hip_failure();
-    if (attr.isManaged) return Status::success;
-    switch (attr.type) {
-    case hipMemoryTypeDevice:
-        return attr.device == device ? Status::success : Status::invalid_argument;
-    case hipMemoryTypeManaged:
-    case hipMemoryTypeUnified:
-        return Status::success;
-    case hipMemoryTypeHost:
-        return attr.devicePointer == data ? Status::success : Status::invalid_argument;
-    default:
-        return pageable ? Status::success : Status::invalid_argument;
-    }
-}
-
-Status stream_device(hipStream_t stream, int device) noexcept {
-    int current = 0;
-    hipDevice_t owner = 0;
-    if (hipGetDevice(&current) != hipSuccess || hipStreamGetDevice(stream, &owner) != hipSuccess)
-        return hip_failure();
-    return current == device && owner == device ? Status::success : Status::invalid_argument;
+    // Callers finish every execution before destroying a plan, so the tables
+    // can go back to the pool without waiting or ordering.
+    if (storage == nullptr) return;
+    static_cast<void>(hip_status(hipFreeAsync(storage, hipStreamPerThread)));
 }
 
 namespace {
 
-Range range_of(const void *data, std::uint64_t size) noexcept {
-    const auto b = reinterpret_cast<std::uintptr_t>(data);
-    return {b, b + static_cast<std::uintptr_t>(size)};
+// Where a pointer's memory lives, as far as the HIP runtime knows.
+enum class Residence { this_device, other_device, managed, mapped_host, unmapped_host, not_registered };
+
+Status residence_of(const void *data, int device, Residence &where) noexcept {
+    hipPointerAttribute_t attr{};
+    const hipError_t e = hipPointerGetAttributes(&attr, data);
+    if (e == hipErrorInvalidValue) {
+        // Ordinary host memory the runtime has never seen.
+        static_cast<void>(hipGetLastError());
+        where = Residence::not_registered;
+        return Status::success;
+    }
+    if (e != hipSuccess) return hip_failure();
+    if (attr.isManaged || attr.type == hipMemoryTypeManaged || attr.type == hipMemoryTypeUnified)
+        where = Residence::managed;
+    else if (attr.type == hipMemoryTypeDevice)
+        where = attr.device == device ? Residence::this_device : Residence::other_device;
+    else if (attr.type == hipMemoryTypeHost)
+        where = attr.devicePointer == data ? Residence::mapped_host : Residence::unmapped_host;
+    else
+        where = Residence::not_registered;
+    return Status::success;
 }
+
+} // namespace
+
+// The plan's device may address the memory: its own device memory, managed
+// memory, pinned host memory mapped at the same address, or any host memory
+// when the device reports pageable memory access (docs/api.md "Alignment and
+// capacity", reference interface plan.cpp:70-85).
+Status accessible(const void *data, int device, bool pageable) noexcept {
+    Residence where = Residence::not_registered;
+    if (const Status st = residence_of(data, device, where); st != Status::success) return st;
+    switch (where) {
+    case Residence::this_device:
+    case Residence::managed:
+    case Residence::mapped_host:
+        return Status::success;
+    case Residence::not_registered:
+        return pageable ? Status::success : Status::invalid_argument;
+    default:
+        return Status::invalid_argument;
+    }
+}
+
+// A call runs on the plan's device: it must be the calling thread's current
+// device and the stream's device.
+Status stream_device(hipStream_t stream, int device) noexcept {
+    int caller = -1;
+    hipDevice_t owner = -1;
+    const bool known = hipGetDevice(&caller) == hipSuccess && hipStreamGetDevice(stream, &owner) == hipSuccess;
+    if (!known) return hip_failure();
+    const bool same = caller == device && owner == device;
+    return same ? Status::success : Status::invalid_argument;
+}
+
+namespace {
 
 std::uint64_t columns(const ImageSpec &image, TileShape tile) noexcept { return host::ceil_div(image.width, tile.width); }
 std::uint64_t tile_rows(const ImageSpec &image, TileShape tile) noexcept { return host::ceil_div(image.height, tile.height); }
 
 // A map matches the output region exactly (reference plan.cpp:174-193).
-Status check_parameter(const Parameter &p, const ImageSpec &region, std::vector<Range> &readable,
+Status check_parameter(const Parameter &p, const ImageSpec &region, std::vector<Span> &readable,
                        std::vector<const void *> &pointers) {
     if (p.kind == ParameterKind::scalar) return Status::success;
     if (p.kind != ParameterKind::map) return Status::invalid_argument;
@@ -108,85 +164,119 @@ Status check_parameter(const Parameter &p, const ImageSpec &region, std::vector<
     if (p.map.image.type != PixelType::float32 || p.map.image.width != region.width ||
         p.map.image.height != region.height)
         return Status::invalid_argument;
-    readable.push_back(range_of(p.map.data, p.map.capacity_bytes));
+    readable.push_back(Span::of(p.map.data, p.map.capacity_bytes));
     pointers.push_back(p.map.data);
     return Status::success;
 }
 
-Status check_calibration(const Calibration &c, const ImageSpec &region, std::vector<Range> &readable,
+Status check_calibration(const Calibration &c, const ImageSpec &region, std::vector<Span> &readable,
                          std::vector<const void *> &pointers) {
     const Status st = check_parameter(c.offset, region, readable, pointers);
     return st != Status::success ? st : check_parameter(c.gain, region, readable, pointers);
 }
 
-} // namespace
+// What docs/format.md fixes for the index entry of tile `index`: the actual
+// size its grid position implies, its RAW length A4(w h s), which bounds
+// every payload, and the widest residual of the pixel type.
+struct TileFacts {
+    std::uint64_t width = 0, height = 0;
+    std::uint64_t raw_length = 0;
+    std::uint32_t widest = 0;
+    bool u16 = true;
+};
 
-// Repeats the parser's structural checks that keep device reads inside the tile
-// payload (reference plan.cpp:238-279).
-Status check_tile(const ImageMetadata &m, std::uint64_t index) noexcept {
-    const TileMetadata &t = m.tiles[static_cast<std::size_t>(index)];
+TileFacts facts_of(const ImageMetadata &m, std::uint64_t index) noexcept {
+    TileFacts f;
     const std::uint64_t across = columns(m.image, m.tile);
     const std::uint64_t x = index % across * m.tile.width, y = index / across * m.tile.height;
-    const std::uint64_t w = std::min<std::uint64_t>(m.tile.width, m.image.width - x);
-    const std::uint64_t h = std::min<std::uint64_t>(m.tile.height, m.image.height - y);
-    const std::uint64_t s = m.image.type == PixelType::uint16 ? 2 : 4;
-    const std::uint32_t widest = m.image.type == PixelType::uint16 ? 16 : 32;
-    const std::uint64_t raw4 = (w * h * s + 3) / 4 * 4;
-    if (t.width != w || t.height != h || t.offset % 4 != 0 || t.offset > m.file_bytes ||
-        t.length > m.file_bytes - t.offset)
-        return Status::malformed_data;
-    switch (t.mode) {
-    case TileMode::raw:
-        return t.length == raw4 ? Status::success : Status::malformed_data;
-    case TileMode::constant:
-        return t.length == 4 ? Status::success : Status::malformed_data;
-    case TileMode::predict_bitpack:
-        break;
-    default:
-        return Status::malformed_data;
+    f.width = std::min<std::uint64_t>(m.tile.width, m.image.width - x);
+    f.height = std::min<std::uint64_t>(m.tile.height, m.image.height - y);
+    f.u16 = m.image.type == PixelType::uint16;
+    f.raw_length = (f.width * f.height * (f.u16 ? 2 : 4) + 3) & ~std::uint64_t{3};
+    f.widest = f.u16 ? 16u : 32u;
+    return f;
+}
+
+} // namespace
+
+// The parser's structural rules that keep device reads inside a tile's
+// payload, checked again for each tile a plan selects (reference interface
+// plan.cpp:238-279): every violation is malformed_data.
+Status check_tile(const ImageMetadata &m, std::uint64_t index) noexcept {
+    const TileMetadata &t = m.tiles[static_cast<std::size_t>(index)];
+    const TileFacts f = facts_of(m, index);
+    // The entry's size agrees with its position; the payload is word aligned
+    // and inside the file.
+    const bool placed = t.width == f.width && t.height == f.height && t.offset % 4 == 0 && t.offset <= m.file_bytes &&
+                        t.length <= m.file_bytes - t.offset;
+    if (!placed) return Status::malformed_data;
+    // RAW and CONSTANT payloads have one exact length each.
+    if (t.mode == TileMode::raw || t.mode == TileMode::constant) {
+        const std::uint64_t exact = t.mode == TileMode::raw ? f.raw_length : 4;
+        return t.length == exact ? Status::success : Status::malformed_data;
     }
-    if (t.length > raw4 || t.first_segment > m.segments.size() || h > m.segments.size() - t.first_segment)
-        return Status::malformed_data;
-    for (std::uint64_t r = 0; r < h; ++r) {
-        const SegmentMetadata &g = m.segments[static_cast<std::size_t>(t.first_segment + r)];
-        if (g.bit_width > widest || (m.image.type == PixelType::uint16 && g.seed > 0xffffu) ||
-            g.payload_offset < format::segment_header_bytes || g.payload_offset % 4 != 0 ||
-            g.payload_offset > t.length || g.payload_bytes > t.length - g.payload_offset ||
-            g.payload_bytes != ((w - 1) * g.bit_width + 31) / 32 * 4)
-            return Status::malformed_data;
-    }
-    return Status::success;
+    if (t.mode != TileMode::predict_bitpack || t.length > f.raw_length) return Status::malformed_data;
+    // A predictive tile owns `height` consecutive row records, each inside the
+    // payload with exactly the packed length its width gives.
+    const std::size_t segments = m.segments.size();
+    if (t.first_segment > segments || f.height > segments - t.first_segment) return Status::malformed_data;
+    const auto row_fits = [&](const SegmentMetadata &g) {
+        const std::uint64_t packed = ((f.width - 1) * g.bit_width + 31) / 32 * 4;
+        const bool header = g.payload_offset >= format::segment_header_bytes && g.payload_offset % 4 == 0;
+        const bool inside = g.payload_offset <= t.length && g.payload_bytes <= t.length - g.payload_offset;
+        return g.bit_width <= f.widest && (!f.u16 || g.seed <= 0xffffu) && header && inside && g.payload_bytes == packed;
+    };
+    const auto first = m.segments.begin() + static_cast<std::ptrdiff_t>(t.first_segment);
+    return std::all_of(first, first + static_cast<std::ptrdiff_t>(f.height), row_fits) ? Status::success
+                                                                                        : Status::malformed_data;
 }
 
 namespace {
 
-// Selects the tiles intersecting the region, rebased into the supplied bytes
-// when the payloads were staged (reference plan.cpp:282-349).
+// The tiles a region intersects: columns [left, right], rows [top, bottom].
+struct TileWindow {
+    std::uint64_t left = 0, right = 0, top = 0, bottom = 0;
+    [[nodiscard]] std::uint64_t count() const noexcept { return (right - left + 1) * (bottom - top + 1); }
+};
+
+TileWindow window_of(const Roi &roi, TileShape tile) noexcept {
+    TileWindow w;
+    w.left = roi.x / tile.width;
+    w.right = (std::uint64_t{roi.x} + roi.width - 1) / tile.width;
+    w.top = roi.y / tile.height;
+    w.bottom = (std::uint64_t{roi.y} + roi.height - 1) / tile.height;
+    return w;
+}
+
+// Selects the tiles intersecting the region, in tile order, each at its
+// file offset or, for staged payloads, at the offset its mapping gives
+// (docs/api.md "Selected-tile uploads": exactly the intersecting tiles in
+// increasing order, 4-byte aligned, inside the supplied bytes, not
+// overlapping one another).
 Status stage_tiles(const CompressedInput &source, const Roi &roi, std::uint32_t image, DecodeDraft &d) {
     const ImageMetadata &m = *source.metadata;
     const std::uint64_t across = columns(m.image, m.tile);
     if (m.tiles.size() != across * tile_rows(m.image, m.tile)) return Status::malformed_data;
-    const std::uint64_t left = roi.x / m.tile.width;
-    const std::uint64_t right = (std::uint64_t{roi.x} + roi.width - 1) / m.tile.width;
-    const std::uint64_t top = roi.y / m.tile.height;
-    const std::uint64_t bottom = (std::uint64_t{roi.y} + roi.height - 1) / m.tile.height;
-    const std::uint64_t selected = (right - left + 1) * (bottom - top + 1);
+    const TileWindow win = window_of(roi, m.tile);
     const bool mapped = source.mappings != nullptr || source.mapping_count != 0;
-    if (mapped ? source.mappings == nullptr || source.mapping_count != selected : m.file_bytes > source.bytes.size)
-        return mapped ? Status::invalid_argument : Status::buffer_too_small;
-    std::vector<Range> staged;
-    std::uint64_t k = 0;
-    for (std::uint64_t row = top; row <= bottom; ++row) {
-        for (std::uint64_t col = left; col <= right; ++col, ++k) {
+    if (mapped && (source.mappings == nullptr || source.mapping_count != win.count())) return Status::invalid_argument;
+    if (!mapped && m.file_bytes > source.bytes.size) return Status::buffer_too_small;
+    const auto mapping_fits = [&](const TileMapping &map, std::uint64_t index, std::uint64_t length) {
+        const std::uint64_t size = source.bytes.size;
+        return map.tile_index == index && map.input_offset % 4 == 0 && map.input_offset <= size &&
+               length <= size - map.input_offset;
+    };
+    std::vector<Span> staged;
+    const TileMapping *next_mapping = source.mappings;
+    for (std::uint64_t row = win.top; row <= win.bottom; ++row) {
+        for (std::uint64_t col = win.left; col <= win.right; ++col) {
             const std::uint64_t index = row * across + col;
             if (const Status st = check_tile(m, index); st != Status::success) return st;
             const TileMetadata &t = m.tiles[static_cast<std::size_t>(index)];
             std::uint64_t offset = t.offset;
             if (mapped) {
-                const TileMapping &map = source.mappings[k];
-                if (map.tile_index != index || map.input_offset % 4 != 0 || map.input_offset > source.bytes.size ||
-                    t.length > source.bytes.size - map.input_offset)
-                    return Status::invalid_argument;
+                const TileMapping &map = *next_mapping++;
+                if (!mapping_fits(map, index, t.length)) return Status::invalid_argument;
                 offset = map.input_offset;
                 staged.push_back({static_cast<std::uintptr_t>(offset), static_cast<std::uintptr_t>(offset + t.length)});
             }
@@ -209,10 +299,10 @@ Status stage_tiles(const CompressedInput &source, const Roi &roi, std::uint32_t 
             d.tasks.push_back(task);
         }
     }
-    std::sort(staged.begin(), staged.end(), [](const Range &a, const Range &b) { return a.begin < b.begin; });
-    for (std::size_t i = 1; i < staged.size(); ++i)
-        if (staged[i].begin < staged[i - 1].end) return Status::invalid_argument;
-    d.tiles_per_image.push_back(selected);
+    // Staged payloads are all written by the caller's copies: none may overlap another.
+    Footprint copies;
+    if (!copies.admit(std::move(staged), {})) return Status::invalid_argument;
+    d.tiles_per_image.push_back(win.count());
     return Status::success;
 }
 
@@ -280,7 +370,7 @@ Status draft_decode_item(const BatchItem &item, Operation operation, bool requir
                 d.tasks.push_back(task);
             }
         d.tiles_per_image.push_back(n);
-        d.readable.push_back(range_of(source.encoder_result, sizeof(DeviceResult)));
+        d.readable.push_back(Span::of(source.encoder_result, sizeof(DeviceResult)));
         d.pointers.push_back(source.encoder_result);
     } else {
         if (const Status st = stage_tiles(source, roi, index, d); st != Status::success) return st;
@@ -319,8 +409,8 @@ Status draft_decode_item(const BatchItem &item, Operation operation, bool requir
         }
     }
     d.images.push_back(rec);
-    d.readable.push_back(range_of(source.bytes.data, source.bytes.size));
-    d.writable.push_back(range_of(output.data, output.capacity_bytes));
+    d.readable.push_back(Span::of(source.bytes.data, source.bytes.size));
+    d.writable.push_back(Span::of(output.data, output.capacity_bytes));
     d.pointers.push_back(source.bytes.data);
     d.pointers.push_back(output.data);
     return Status::success;
@@ -390,8 +480,8 @@ struct Draft {
     std::vector<std::uint64_t> first_rows;
     std::uint64_t total_rows = 0;
     std::vector<std::uint64_t> tiles; // tiles per image, for windows
-    std::vector<Range> writable;
-    std::vector<Range> readable;
+    std::vector<Span> writable;
+    std::vector<Span> readable;
     std::vector<const void *> pointers;
 };
 
@@ -421,8 +511,8 @@ Status draft_compress(const BatchRequest &request, Draft &d) {
         rec.type = static_cast<std::uint32_t>(item.input.image.type);
         d.encode.push_back(rec);
         d.tiles.push_back(rec.tile_count);
-        d.readable.push_back(range_of(item.input.data, item.input.capacity_bytes));
-        d.writable.push_back(range_of(out.data, out.size));
+        d.readable.push_back(Span::of(item.input.data, item.input.capacity_bytes));
+        d.writable.push_back(Span::of(out.data, out.size));
         d.pointers.push_back(item.input.data);
         d.pointers.push_back(out.data);
     }
@@ -462,8 +552,8 @@ Status draft_calibrate(const BatchRequest &request, Draft &d) {
         d.calibrate.push_back(rec);
         d.first_rows.push_back(d.total_rows);
         d.total_rows += item.input.image.height;
-        d.readable.push_back(range_of(item.input.data, item.input.capacity_bytes));
-        d.writable.push_back(range_of(item.output.data, item.output.capacity_bytes));
+        d.readable.push_back(Span::of(item.input.data, item.input.capacity_bytes));
+        d.writable.push_back(Span::of(item.output.data, item.output.capacity_bytes));
         d.pointers.push_back(item.input.data);
         d.pointers.push_back(item.output.data);
     }
@@ -484,36 +574,6 @@ Status draft_decode(const BatchRequest &request, Draft &d) {
     return Status::success;
 }
 
-// Splits the concatenated tiles of all images into windows of at most
-// window_tiles tiles (reference plan.cpp:468-491).
-struct WindowPlan {
-    std::vector<tsk_window_entry> entries;
-    std::vector<std::pair<std::size_t, EncodeWindow>> windows; // first entry, window
-};
-
-WindowPlan split_windows(const std::vector<std::uint64_t> &tiles) {
-    WindowPlan plan;
-    std::uint32_t used = 0;
-    for (std::size_t image = 0; image < tiles.size(); ++image) {
-        for (std::uint64_t start = 0; start < tiles[image];) {
-            if (plan.windows.empty() || used == window_tiles) {
-                plan.windows.push_back({plan.entries.size(), {}});
-                used = 0;
-            }
-            const auto take = static_cast<std::uint32_t>(std::min<std::uint64_t>(tiles[image] - start, window_tiles - used));
-            std::uint32_t flags = start == 0 ? 0u : 1u;
-            if (start + take == tiles[image]) flags |= 2u;
-            plan.entries.push_back({static_cast<std::uint32_t>(image), used, take, flags, start});
-            EncodeWindow &w = plan.windows.back().second;
-            ++w.entry_count;
-            w.tile_count += take;
-            used += take;
-            start += take;
-        }
-    }
-    return plan;
-}
-
 template <typename T> bool upload(void *storage, std::uint64_t at, const std::vector<T> &v, hipStream_t stream) {
     return v.empty() || hipMemcpyAsync(static_cast<std::uint8_t *>(storage) + at, v.data(), v.size() * sizeof(T),
                                        hipMemcpyHostToDevice, stream) == hipSuccess;
@@ -530,7 +590,7 @@ Status build(const BatchRequest &request, PlanState &plan, hipStream_t stream) {
                                                             : draft_decode(request, d);
     if (st != Status::success) return st;
     plan.type = d.type;
-    if (!plan.ranges.assign(std::move(d.writable), std::move(d.readable))) return Status::invalid_argument;
+    if (!plan.footprint.admit(std::move(d.writable), std::move(d.readable))) return Status::invalid_argument;
 
     int pageable = 0;
     if (hipGetDevice(&plan.device) != hipSuccess ||
@@ -541,7 +601,7 @@ Status build(const BatchRequest &request, PlanState &plan, hipStream_t stream) {
     for (const void *p : d.pointers)
         if ((st = accessible(p, plan.device, plan.pageable)) != Status::success) return st;
 
-    WindowPlan windows = split_windows(d.tiles);
+    WindowSplit windows = split_into_windows(d.tiles, window_tiles);
     // One window over the whole batch when its look-back buffers (2 x 8 bytes
     // per tile) fit the residual region the spec reserves (16 KiB per window tile).
     std::uint64_t total_tiles = 0;
@@ -561,16 +621,16 @@ Status build(const BatchRequest &request, PlanState &plan, hipStream_t stream) {
     // Tile-to-entry tables: one per window, then the whole-batch window's.
     std::vector<std::uint32_t> blocks;
     std::vector<std::size_t> blocks_first;
-    for (const auto &[first, w] : windows.windows) {
+    for (const WindowSpan &w : windows.windows) {
         blocks_first.push_back(blocks.size());
-        append_block_owners(windows.entries.data() + first, w.entry_count, w.tile_count, blocks);
+        append_block_owners(windows.entries.data() + w.first_entry, w.entry_count, w.tile_count, blocks);
     }
     const std::size_t whole_blocks_at = blocks.size();
     if (whole)
         append_block_owners(windows.entries.data() + whole_at, d.tiles.size(), static_cast<std::uint32_t>(total_tiles),
                             blocks);
     plan.spec = {0, 0, d.shape, 0};
-    for (const auto &[first, w] : windows.windows) {
+    for (const WindowSpan &w : windows.windows) {
         plan.spec.image_count = std::max<std::uint64_t>(plan.spec.image_count, w.entry_count);
         plan.spec.tile_count = std::max<std::uint64_t>(plan.spec.tile_count, w.tile_count);
     }
@@ -618,9 +678,12 @@ Status build(const BatchRequest &request, PlanState &plan, hipStream_t stream) {
     plan.first_rows = reinterpret_cast<const std::uint64_t *>(base + first_at);
     plan.total_rows = d.total_rows;
     for (std::size_t k = 0; k < windows.windows.size(); ++k) {
-        EncodeWindow placed = windows.windows[k].second;
-        placed.entries = reinterpret_cast<const tsk_window_entry *>(base + entries_at) + windows.windows[k].first;
+        const WindowSpan &w = windows.windows[k];
+        EncodeWindow placed;
+        placed.entries = reinterpret_cast<const tsk_window_entry *>(base + entries_at) + w.first_entry;
         placed.blocks = reinterpret_cast<const std::uint32_t *>(base + blocks_at) + blocks_first[k];
+        placed.entry_count = w.entry_count;
+        placed.tile_count = w.tile_count;
         plan.windows.push_back(placed);
     }
     if (whole) {
@@ -650,33 +713,61 @@ Status build(const BatchRequest &request, PlanState &plan, hipStream_t stream) {
 
 } // namespace
 
+namespace {
+
+// Evaluates the checks in order and returns the first failure.
+template <typename... Checks> Status first_failure(Checks &&...checks) {
+    Status st = Status::success;
+    static_cast<void>(((st = checks(), st == Status::success) && ...));
+    return st;
+}
+
+} // namespace
+
+Status check_call(const CallNeeds &needs, MutableBytes workspace, ResultBuffer results, hipStream_t stream) noexcept {
+    std::uint64_t result_bytes = 0;
+    const Span scratch = Span::of(workspace.data, workspace.size);
+    return first_failure(
+        // Both byte ranges are representable, and non-null when non-empty.
+        [&] { return host::mul(results.count, sizeof(DeviceResult), result_bytes) ? Status::success : Status::size_overflow; },
+        [&] { return byte_range(workspace.data, workspace.size); },
+        [&] { return byte_range(results.data, result_bytes); },
+        // Room for the plan's workspace and one record per image.
+        [&] {
+            const bool room = workspace.size >= needs.workspace && results.count >= needs.results;
+            return room ? Status::success : Status::buffer_too_small;
+        },
+        // Workspace on a 256-byte boundary, records naturally aligned.
+        [&] {
+            const bool aligned = reinterpret_cast<std::uintptr_t>(workspace.data) % workspace_alignment == 0 &&
+                                 reinterpret_cast<std::uintptr_t>(results.data) % alignof(DeviceResult) == 0;
+            return aligned ? Status::success : Status::invalid_argument;
+        },
+        // Both are written: they overlap no buffer of the batch and not each other.
+        [&] {
+            const Span records = Span::of(results.data, result_bytes);
+            const bool apart = needs.footprint->avoids(scratch) && needs.footprint->avoids(records) &&
+                               !host::overlaps(workspace.data, workspace.size, results.data, result_bytes);
+            return apart ? Status::success : Status::invalid_argument;
+        },
+        [&] { return stream_device(stream, needs.device); },
+        // An empty workspace is never touched, so its pointer need not be device memory.
+        [&] { return workspace.size == 0 ? Status::success : accessible(workspace.data, needs.device, needs.pageable); },
+        [&] { return accessible(results.data, needs.device, needs.pageable); },
+        // Launch failures surface only as the runtime's last error: one left by
+        // the caller's earlier work would be reported as this call's, so it is
+        // left for the caller and nothing is enqueued.
+        [] { return hipPeekAtLastError() == hipSuccess ? Status::success : Status::hip_error; });
+}
+
 // Host checks of every execution call, before anything is enqueued
-// (reference plan.cpp:627-674).
+// (reference interface plan.cpp:627-674, docs/api.md "Status and completion").
 Status check_execution(const PlanState *plan, Operation operation, MutableBytes workspace, ResultBuffer results,
                        hipStream_t stream) noexcept {
     if (plan == nullptr || plan->operation != operation) return Status::invalid_argument;
-    if (plan->count == 0) return Status::success;
-    std::uint64_t result_bytes = 0;
-    if (!host::mul(results.count, sizeof(DeviceResult), result_bytes)) return Status::size_overflow;
-    Status st = byte_range(workspace.data, workspace.size);
-    if (st == Status::success) st = byte_range(results.data, result_bytes);
-    if (st != Status::success) return st;
-    if (workspace.size < plan->layout.total || results.count < plan->count) return Status::buffer_too_small;
-    if (reinterpret_cast<std::uintptr_t>(workspace.data) % workspace_alignment != 0 ||
-        reinterpret_cast<std::uintptr_t>(results.data) % alignof(DeviceResult) != 0)
-        return Status::invalid_argument;
-    if (!plan->ranges.disjoint(range_of(workspace.data, workspace.size)) ||
-        !plan->ranges.disjoint(range_of(results.data, result_bytes)) ||
-        host::overlaps(workspace.data, workspace.size, results.data, result_bytes))
-        return Status::invalid_argument;
-    st = stream_device(stream, plan->device);
-    // An empty workspace is never touched, so its pointer need not be device memory.
-    if (st == Status::success && workspace.size != 0) st = accessible(workspace.data, plan->device, plan->pageable);
-    if (st == Status::success) st = accessible(results.data, plan->device, plan->pageable);
-    // A pending error from the caller's earlier work would be reported as ours:
-    // leave it for the caller and enqueue nothing.
-    if (st == Status::success && hipPeekAtLastError() != hipSuccess) st = Status::hip_error;
-    return st;
+    if (plan->count == 0) return Status::success; // an empty batch touches no buffer
+    const CallNeeds needs{&plan->footprint, plan->device, plan->pageable, plan->count, plan->layout.total};
+    return check_call(needs, workspace, results, stream);
 }
 
 Status run_decode(const PlanState &plan, DeviceResult *results, hipStream_t stream, const DecodeShape &shape) noexcept {
@@ -807,20 +898,12 @@ Status compress_async(const BatchPlan &plan, MutableBytes workspace, ResultBuffe
 
 Status compress_async_static(const BatchPlan &plan, ResultBuffer results, hipStream_t stream) noexcept {
     const detail::PlanState *state = plan.impl();
-    // The checks of compress_async for everything but the workspace.
+    // The checks of compress_async, with no workspace.
     if (state == nullptr || state->operation != Operation::compress) return Status::invalid_argument;
     if (state->count == 0) return Status::success;
-    std::uint64_t result_bytes = 0;
-    if (!host::mul(results.count, sizeof(DeviceResult), result_bytes)) return Status::size_overflow;
-    if (const Status st = host::byte_range(results.data, result_bytes); st != Status::success) return st;
-    if (results.count < state->count) return Status::buffer_too_small;
-    if (reinterpret_cast<std::uintptr_t>(results.data) % alignof(DeviceResult) != 0) return Status::invalid_argument;
-    const auto at = reinterpret_cast<std::uintptr_t>(results.data);
-    if (!state->ranges.disjoint({at, at + static_cast<std::uintptr_t>(result_bytes)})) return Status::invalid_argument;
-    Status st = detail::stream_device(stream, state->device);
-    if (st == Status::success) st = detail::accessible(results.data, state->device, state->pageable);
-    if (st == Status::success && hipPeekAtLastError() != hipSuccess) st = Status::hip_error;
-    if (st != Status::success) return st;
+    const detail::CallNeeds needs{&state->footprint, state->device, state->pageable, state->count, 0};
+    if (const Status st = detail::check_call(needs, MutableBytes{nullptr, 0}, results, stream); st != Status::success)
+        return st;
 
     void *scratch = nullptr;
     std::uint64_t scratch_bytes = 0;

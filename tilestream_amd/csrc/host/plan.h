@@ -15,23 +15,48 @@ namespace tilestream::hip::detail {
 // Tiles per encode window: bounds the encoder's workspace for any batch size.
 inline constexpr std::uint32_t window_tiles = 4096;
 
-struct Range {
-    std::uintptr_t begin = 0;
-    std::uintptr_t end = 0;
+// A half-open address interval [lo, hi).
+struct Span {
+    std::uintptr_t lo = 0;
+    std::uintptr_t hi = 0;
+    [[nodiscard]] bool empty() const noexcept { return lo == hi; }
+    [[nodiscard]] static Span of(const void *data, std::uint64_t bytes) noexcept {
+        const auto at = reinterpret_cast<std::uintptr_t>(data);
+        return {at, at + static_cast<std::uintptr_t>(bytes)};
+    }
 };
 
-// Writable ranges are pairwise disjoint and disjoint from every readable range;
-// readable ranges may overlap one another (reference src/cuda/plan.cpp:21-61).
-class RangeSet {
+// The memory a prepared batch touches, checked against the API's aliasing
+// rule: a buffer the batch writes overlaps no other buffer of the batch,
+// while buffers it only reads may overlap each other.  Kept as the sorted
+// union of all of them, so a buffer an execution call adds (workspace,
+// results) is checked with one binary search.
+class Footprint {
   public:
-    [[nodiscard]] bool assign(std::vector<Range> writable, std::vector<Range> readable);
-    [[nodiscard]] bool disjoint(Range range) const noexcept;
+    // Records the batch's buffers; false if the rule is broken.
+    [[nodiscard]] bool admit(std::vector<Span> written, std::vector<Span> read);
+    // Whether `s` overlaps none of the recorded buffers.
+    [[nodiscard]] bool avoids(Span s) const noexcept;
 
   private:
-    std::vector<Range> writable_;
-    std::vector<Range> readable_;
-    std::vector<std::uintptr_t> reach_; // running maximum end of the sorted readable ranges
+    std::vector<Span> union_; // sorted, coalesced
 };
+
+// Splits the concatenated tiles of consecutive images (tiles[i] of image i)
+// into windows of at most `window` tiles: window k covers tiles
+// [k window, (k + 1) window) of the concatenation, and holds one entry per
+// image it intersects (window-relative first tile, count, image-relative
+// start, flags 1: continues an earlier window, 2: finishes the image).
+struct WindowSpan {
+    std::size_t first_entry = 0;
+    std::uint32_t entry_count = 0;
+    std::uint32_t tile_count = 0;
+};
+struct WindowSplit {
+    std::vector<tsk_window_entry> entries;
+    std::vector<WindowSpan> windows;
+};
+[[nodiscard]] WindowSplit split_into_windows(const std::vector<std::uint64_t> &tiles, std::uint32_t window);
 
 struct EncodeWindow {
     const tsk_window_entry *entries = nullptr; // device
@@ -68,7 +93,7 @@ struct PlanState {
     std::uint64_t count = 0;
     WorkspaceSpec spec;
     host::WorkspaceLayout layout;
-    RangeSet ranges;
+    Footprint footprint;
     void *storage = nullptr; // one device allocation holding every table
 
     // decode / fused decode
@@ -112,6 +137,21 @@ inline Status hip_status(int error) noexcept { return error == hipSuccess ? Stat
 [[nodiscard]] Status accessible(const void *data, int device, bool pageable) noexcept;
 [[nodiscard]] Status check_tile(const ImageMetadata &metadata, std::uint64_t index) noexcept;
 [[nodiscard]] Status stream_device(hipStream_t stream, int device) noexcept;
+
+// What a prepared batch asks of the buffers and stream of an execution call.
+struct CallNeeds {
+    const Footprint *footprint = nullptr;
+    int device = 0;
+    bool pageable = false;
+    std::uint64_t results = 0;   // result records
+    std::uint64_t workspace = 0; // workspace bytes
+};
+// The host checks of an execution call, before anything is enqueued, in the
+// order the reference API reports them: representable ranges, capacities,
+// alignment, aliasing with the batch and each other, the stream's device,
+// where the buffers live, and a runtime error left pending by the caller.
+[[nodiscard]] Status check_call(const CallNeeds &needs, MutableBytes workspace, ResultBuffer results,
+                                hipStream_t stream) noexcept;
 [[nodiscard]] Status check_execution(const PlanState *plan, Operation operation, MutableBytes workspace,
                                      ResultBuffer results, hipStream_t stream) noexcept;
 [[nodiscard]] Status run_decode(const PlanState &plan, DeviceResult *results, hipStream_t stream,
@@ -124,8 +164,8 @@ struct DecodeDraft {
     std::vector<tsk_decode_task> tasks;
     std::vector<std::uint16_t> rows;
     std::vector<std::uint64_t> tiles_per_image;
-    std::vector<Range> writable;
-    std::vector<Range> readable;
+    std::vector<Span> writable;
+    std::vector<Span> readable;
     std::vector<const void *> pointers;
     bool mixed_shapes = false;
     bool first = true;

@@ -16,7 +16,7 @@ struct DecodeQueue::Impl {
     std::uint64_t count = 0;
     bool has_calibrated = false, has_plain = false;
     bool has_u16 = false, has_f32 = false;
-    detail::RangeSet ranges;
+    detail::Footprint footprint;
     void *storage = nullptr;
     const tsk_decode_image *images = nullptr;
     const tsk_decode_task *tasks = nullptr;
@@ -74,7 +74,7 @@ Status DecodeQueue::prepare(const BatchItem *items, const Operation *operations,
                 return Status::invalid_argument;
             if (const Status st = detail::draft_decode_item(items[i], op, false, d); st != Status::success) return st;
         }
-        if (!impl->ranges.assign(std::move(d.writable), std::move(d.readable))) return Status::invalid_argument;
+        if (!impl->footprint.admit(std::move(d.writable), std::move(d.readable))) return Status::invalid_argument;
         QueueStats &s = impl->stats;
         s.images = count;
         s.tasks = d.tasks.size();
@@ -147,21 +147,9 @@ void DecodeQueue::set_shape(const LaunchShape &shape) noexcept { impl_->shape = 
 Status DecodeQueue::run(MutableBytes workspace, ResultBuffer results, hipStream_t stream) noexcept {
     Impl &q = *impl_;
     if (q.count == 0) return Status::success;
-    std::uint64_t result_bytes = 0;
-    if (!host::mul(results.count, sizeof(DeviceResult), result_bytes)) return Status::size_overflow;
-    if (const Status st = host::byte_range(workspace.data, workspace.size); st != Status::success) return st;
-    if (const Status st = host::byte_range(results.data, result_bytes); st != Status::success) return st;
-    if (workspace.size < kCounterBytes || results.count < q.count) return Status::buffer_too_small;
-    if (reinterpret_cast<std::uintptr_t>(workspace.data) % workspace_alignment != 0 ||
-        reinterpret_cast<std::uintptr_t>(results.data) % alignof(DeviceResult) != 0)
-        return Status::invalid_argument;
-    const detail::Range ws{reinterpret_cast<std::uintptr_t>(workspace.data),
-                           reinterpret_cast<std::uintptr_t>(workspace.data) + workspace.size};
-    const detail::Range rs{reinterpret_cast<std::uintptr_t>(results.data),
-                           reinterpret_cast<std::uintptr_t>(results.data) + result_bytes};
-    if (!q.ranges.disjoint(ws) || !q.ranges.disjoint(rs) ||
-        host::overlaps(workspace.data, workspace.size, results.data, result_bytes))
-        return Status::invalid_argument;
+    // The same host checks as an execution of a plan.
+    const detail::CallNeeds needs{&q.footprint, q.device, q.pageable, q.count, kCounterBytes};
+    if (const Status st = detail::check_call(needs, workspace, results, stream); st != Status::success) return st;
     const LaunchShape &sh = q.shape;
     // Group 0 selects the block decoder, which needs every payload address on
     // the host (not the case for encoder output decoded from its device index).

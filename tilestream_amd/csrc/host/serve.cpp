@@ -240,29 +240,6 @@ struct Extent {
     std::uint64_t staged = 0;  // region of a host archive: staging bytes (16-byte blocks)
 };
 
-// Splits the tiles of consecutive images into windows of at most `window`
-// tiles; records the entry index where each window starts.
-void split_windows(const std::vector<std::uint64_t> &tiles, std::uint32_t window, std::vector<tsk_window_entry> &entries,
-                   std::vector<std::pair<std::size_t, std::uint32_t>> &windows) {
-    std::uint32_t used = window;
-    for (std::size_t image = 0; image < tiles.size(); ++image) {
-        for (std::uint64_t start = 0; start < tiles[image];) {
-            if (used == window) {
-                windows.push_back({entries.size(), 0});
-                used = 0;
-            }
-            const auto take = static_cast<std::uint32_t>(
-                std::min<std::uint64_t>(tiles[image] - start, window - used));
-            std::uint32_t flags = start == 0 ? 0u : 1u;
-            if (start + take == tiles[image]) flags |= 2u;
-            entries.push_back({static_cast<std::uint32_t>(image), used, take, flags, start});
-            windows.back().second += take;
-            used += take;
-            start += take;
-        }
-    }
-}
-
 } // namespace
 
 struct Server::Impl {
@@ -443,20 +420,19 @@ Status Server::Impl::launch(Slot &slot, const ServeRequest *requests, const std:
             enc_tiles[g].push_back(extents[ids[k]].tasks);
         }
     }
-    std::vector<tsk_window_entry> entries;
-    std::vector<std::pair<std::size_t, std::uint32_t>> windows[2];
-    split_windows(enc_tiles[0], options.encode_window, entries, windows[0]);
+    // Encode windows per pixel type; the entries of both groups in one table.
+    detail::WindowSplit split[2] = {detail::split_into_windows(enc_tiles[0], options.encode_window),
+                                    detail::split_into_windows(enc_tiles[1], options.encode_window)};
+    std::vector<tsk_window_entry> entries = std::move(split[0].entries);
     const std::size_t group1_entries = entries.size();
-    split_windows(enc_tiles[1], options.encode_window, entries, windows[1]);
+    entries.insert(entries.end(), split[1].entries.begin(), split[1].entries.end());
+    for (detail::WindowSpan &w : split[1].windows) w.first_entry += group1_entries;
     std::vector<std::uint32_t> blocks;
     std::vector<std::size_t> blocks_first[2];
     for (int g = 0; g < 2; ++g) {
-        const std::size_t end_entries = g == 0 ? group1_entries : entries.size();
-        for (std::size_t w = 0; w < windows[g].size(); ++w) {
-            const std::size_t e0 = windows[g][w].first;
-            const std::size_t e1 = w + 1 < windows[g].size() ? windows[g][w + 1].first : end_entries;
+        for (const detail::WindowSpan &w : split[g].windows) {
             blocks_first[g].push_back(blocks.size());
-            detail::append_block_owners(entries.data() + e0, e1 - e0, windows[g][w].second, blocks);
+            detail::append_block_owners(entries.data() + w.first_entry, w.entry_count, w.tile_count, blocks);
         }
     }
     const Layout l = layout_for(regions, encodes[0] + encodes[1], tasks, entries.size(), blocks.size());
@@ -609,14 +585,12 @@ Status Server::Impl::launch(Slot &slot, const ServeRequest *requests, const std:
             a.images = reinterpret_cast<const tsk_encode_image *>(slot.device + l.encode) + (g == 0 ? 0 : encodes[0]);
             a.results = slot.device + l.results + (regions + (g == 0 ? 0 : encodes[0])) * sizeof(DeviceResult);
             a.type = g == 0 ? TSK_UINT16 : TSK_FLOAT32;
-            const std::size_t end_entries = g == 0 ? group1_entries : entries.size();
-            for (std::size_t w = 0; w < windows[g].size(); ++w) {
-                const std::size_t e0 = windows[g][w].first;
-                const std::size_t e1 = w + 1 < windows[g].size() ? windows[g][w + 1].first : end_entries;
-                a.entries = reinterpret_cast<const tsk_window_entry *>(slot.device + l.entries) + e0;
+            for (std::size_t w = 0; w < split[g].windows.size(); ++w) {
+                const detail::WindowSpan &span = split[g].windows[w];
+                a.entries = reinterpret_cast<const tsk_window_entry *>(slot.device + l.entries) + span.first_entry;
                 a.blocks = reinterpret_cast<const std::uint32_t *>(slot.device + l.blocks) + blocks_first[g][w];
-                a.entry_count = static_cast<std::uint32_t>(e1 - e0);
-                a.tile_count = windows[g][w].second;
+                a.entry_count = span.entry_count;
+                a.tile_count = span.tile_count;
                 a.parity = sequence++ & 1u;
                 if (tsk_encode_window(&a, slot.stream) != hipSuccess) return detail::hip_failure();
             }
@@ -644,7 +618,7 @@ Status Server::replay(const ServeRequest *requests, const std::uint64_t *arrival
     try {
         // Validate everything before the clock starts.
         s.extents.assign(count, Extent{});
-        std::vector<detail::Range> writable, readable;
+        std::vector<detail::Span> writable, readable;
         writable.reserve(count);
         for (std::uint64_t i = 0; i < count; ++i) {
             if (i > 0 && arrival_ns[i] < arrival_ns[i - 1]) return Status::invalid_argument;
@@ -673,8 +647,8 @@ Status Server::replay(const ServeRequest *requests, const std::uint64_t *arrival
             const auto at = reinterpret_cast<std::uintptr_t>(a.bytes);
             readable.push_back({at, at + a.info.file_bytes});
         }
-        detail::RangeSet ranges;
-        if (!ranges.assign(std::move(writable), std::move(readable))) return Status::invalid_argument;
+        detail::Footprint footprint;
+        if (!footprint.admit(std::move(writable), std::move(readable))) return Status::invalid_argument;
 
         using clock = std::chrono::steady_clock;
         const auto start = clock::now();
