@@ -161,24 +161,35 @@ Status walk_all(const std::uint8_t *d, std::uint64_t size, const Geometry &g, co
     const std::uint64_t threads =
         std::min<std::uint64_t>({std::uint64_t{hw}, 16, size / kBytesPerThread + 1, (g.count + 63) / 64});
     if (threads <= 1) return walk_payloads(d, g, first, 0, g.count, out);
+    const std::uint64_t per = (g.count + threads - 1) / threads;
+    const auto range = [&](std::uint64_t k) {
+        return walk_payloads(d, g, first, std::min(g.count, k * per), std::min(g.count, (k + 1) * per), out);
+    };
+    std::vector<Status> status;
+    std::vector<std::thread> pool;
     try {
-        std::vector<Status> status(static_cast<std::size_t>(threads), Status::success);
-        std::vector<std::thread> pool;
+        // Sized up front: starting a worker never reallocates what the
+        // running ones write to.
+        status.assign(static_cast<std::size_t>(threads), Status::success);
         pool.reserve(static_cast<std::size_t>(threads - 1));
-        const std::uint64_t per = (g.count + threads - 1) / threads;
-        for (std::uint64_t k = 1; k < threads; ++k)
-            pool.emplace_back([&, k] {
-                status[static_cast<std::size_t>(k)] =
-                    walk_payloads(d, g, first, std::min(g.count, k * per), std::min(g.count, (k + 1) * per), out);
-            });
-        status[0] = walk_payloads(d, g, first, 0, std::min(g.count, per), out);
-        for (std::thread &t : pool) t.join();
-        for (const Status st : status)
-            if (st != Status::success) return st;
-        return Status::success;
-    } catch (const std::system_error &) {
-        return walk_payloads(d, g, first, 0, g.count, out); // no threads available: one pass here
+    } catch (const std::bad_alloc &) {
+        return Status::resource_limit;
+    } catch (const std::length_error &) {
+        return Status::resource_limit;
     }
+    try {
+        for (std::uint64_t k = 1; k < threads; ++k)
+            pool.emplace_back([&status, &range, k] { status[static_cast<std::size_t>(k)] = range(k); });
+    } catch (const std::system_error &) {
+        // Fewer threads than asked for: the ranges left without one run below.
+    }
+    const std::uint64_t started = pool.size();
+    status[0] = range(0);
+    for (std::uint64_t k = started + 1; k < threads; ++k) status[static_cast<std::size_t>(k)] = range(k);
+    for (std::thread &t : pool) t.join();
+    for (const Status st : status)
+        if (st != Status::success) return st;
+    return Status::success;
 }
 
 Status native_bytes(std::uint64_t tiles, std::uint64_t segments, std::uint64_t &bytes) noexcept {
