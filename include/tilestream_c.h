@@ -92,6 +92,10 @@ uint32_t ts_metadata_get_tile(const ts_metadata *metadata, uint64_t index, ts_ti
 uint32_t ts_metadata_set_tile(ts_metadata *metadata, uint64_t index, const ts_tile_metadata *in);
 uint32_t ts_metadata_get_segment(const ts_metadata *metadata, uint64_t index, ts_segment_metadata *out);
 uint32_t ts_metadata_set_segment(ts_metadata *metadata, uint64_t index, const ts_segment_metadata *in);
+/* Keeps the first `tiles` tile and `segments` segment records (no more than
+ * there are), as resizing the C++ ImageMetadata vectors would. */
+uint32_t ts_metadata_truncate(ts_metadata *metadata, uint64_t tiles, uint64_t segments);
+void ts_metadata_set_file_bytes(ts_metadata *metadata, uint64_t file_bytes);
 uint32_t ts_cpu_compress(ts_const_image_view input, ts_tile_shape tile, ts_mutable_bytes output,
                          uint64_t *bytes_written);
 uint32_t ts_cpu_decompress(ts_const_bytes input, ts_image_view output, const ts_parse_limits *limits);

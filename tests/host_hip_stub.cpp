@@ -97,7 +97,11 @@ void check_tile_task(const tsk_decode_args &a, const tsk_decode_task &t, const t
     check(t.column < img.tile_columns, "tile column");
     if (t.payload != nullptr) {
         check(t.payload >= img.src && t.payload + std::uint64_t{t.words} * 4 <= img.src + img.src_size, "payload span");
-        if (mode == 2 && t.rows != TSK_NO_ROWS) check(std::uint64_t{t.rows} + h <= a.row_count, "row table");
+        if (mode == 2 && t.rows != TSK_NO_ROWS) {
+            check(std::uint64_t{t.rows} + h <= a.row_count, "row table");
+            // Each row's descriptor (seed and width words) inside the payload.
+            for (std::uint32_t r = 0; r < h; ++r) check(a.rows[t.rows + r] + 2u <= t.words, "row descriptor");
+        }
     }
 }
 

@@ -261,6 +261,91 @@ void trusted_and_queue(std::mt19937 &rng) {
     }
 }
 
+// One field of parsed metadata changed (the mutation classes of reference
+// tests/cuda_robustness_test.cpp:196-240): preparation rejects the result
+// with a format status, or every table entry of the decode it plans is in
+// bounds (the stand-in's launcher checks).
+std::uint64_t nudge(std::uint64_t v, std::mt19937_64 &rng, std::uint64_t step = 1) {
+    if (rng() % 3 == 0) return rng() % 3 == 0 ? rng() : rng() % 65536;
+    return v + (rng() % 17 - 8) * step;
+}
+
+void fabricated(std::mt19937 &seed_rng) {
+    std::mt19937_64 rng(seed_rng());
+    const TileShape shapes[3] = {{128, 32}, {64, 64}, {256, 16}};
+    std::uint64_t accepted = 0, malformed = 0;
+    for (PixelType type : {PixelType::uint16, PixelType::float32}) {
+        for (const TileShape shape : shapes) {
+            // Constant, smooth and noisy bands: all three tile modes.
+            const std::uint32_t s = type == PixelType::uint16 ? 2 : 4;
+            std::vector<std::uint8_t> px(300 * 192 * s);
+            for (std::size_t i = 0; i < px.size(); ++i) {
+                const std::size_t row = i / s / 300;
+                px[i] = static_cast<std::uint8_t>(row < 64 ? 7 : row < 128 ? (i / s) % 300 / 8 : rng());
+            }
+            std::uint64_t bound = 0;
+            EXPECT(compress_bound(ImageSpec{type, 300, 192}, shape, bound) == Status::success);
+            std::vector<std::uint8_t> file(bound);
+            std::uint64_t written = 0;
+            EXPECT(cpu::compress(ConstImageView{px.data(), ImageSpec{type, 300, 192}, 300ull * s, px.size()}, shape,
+                                 MutableBytes{file.data(), file.size()}, written) == Status::success);
+            file.resize(written);
+            ImageMetadata parsed;
+            EXPECT(parse_metadata(ConstBytes{file.data(), file.size()}, ParseLimits{}, parsed) == Status::success);
+            void *src = device_alloc(file.size());
+            std::memcpy(src, file.data(), file.size());
+            void *out = device_alloc(300 * 192 * 4);
+            auto *results = static_cast<DeviceResult *>(device_alloc(16));
+            constexpr std::uint64_t kWorkspace = 1 << 20;
+            void *workspace = device_alloc(kWorkspace);
+            for (int trial = 0; trial < 500; ++trial) {
+                ImageMetadata m = parsed;
+                TileMetadata &t = m.tiles[rng() % m.tiles.size()];
+                SegmentMetadata &g = m.segments.empty() ? m.segments.emplace_back() : m.segments[rng() % m.segments.size()];
+                switch (rng() % 12) {
+                case 0: t.mode = static_cast<TileMode>(rng() % 4); break;
+                case 1: t.width = static_cast<std::uint16_t>(nudge(t.width, rng)); break;
+                case 2: t.height = static_cast<std::uint16_t>(nudge(t.height, rng)); break;
+                case 3: t.offset = nudge(t.offset, rng, 4); break;
+                case 4: t.length = nudge(t.length, rng, 4); break;
+                case 5: t.first_segment = nudge(t.first_segment, rng); break;
+                case 6: g.seed = static_cast<std::uint32_t>(nudge(g.seed, rng)); break;
+                case 7: g.bit_width = static_cast<std::uint32_t>(rng() % 40); break;
+                case 8: g.payload_offset = nudge(g.payload_offset, rng, 4); break;
+                case 9: g.payload_bytes = nudge(g.payload_bytes, rng, 4); break;
+                case 10: m.file_bytes = nudge(m.file_bytes, rng, 4); break;
+                default:
+                    if (rng() % 2 == 0) m.tiles.pop_back();
+                    else if (!m.segments.empty()) m.segments.pop_back();
+                    break;
+                }
+                BatchItem it;
+                it.compressed_input.bytes = ConstBytes{src, file.size()};
+                it.compressed_input.metadata = &m;
+                const auto x = static_cast<std::uint32_t>(rng() % 300), y = static_cast<std::uint32_t>(rng() % 192);
+                it.roi = Roi{x, y, 1 + static_cast<std::uint32_t>(rng() % (300 - x)),
+                             1 + static_cast<std::uint32_t>(rng() % (192 - y))};
+                it.output = ImageView{out, ImageSpec{type, it.roi.width, it.roi.height}, std::uint64_t{it.roi.width} * s,
+                                      300 * 192 * 4};
+                std::unique_ptr<BatchPlan> plan;
+                const Status st = prepare_batch(BatchRequest{Operation::decode_roi, &it, 1, {}}, plan, nullptr);
+                if (st != Status::success) {
+                    EXPECT(st == Status::malformed_data || st == Status::unsupported_format ||
+                           st == Status::buffer_too_small);
+                    malformed += st == Status::malformed_data;
+                    continue;
+                }
+                ++accepted;
+                std::uint64_t ws = 0;
+                EXPECT(workspace_size(plan->workspace_spec(), ws) == Status::success && ws <= kWorkspace);
+                EXPECT(decode_roi_async(*plan, MutableBytes{workspace, ws}, ResultBuffer{results, 1}, nullptr) ==
+                       Status::success);
+            }
+        }
+    }
+    EXPECT(accepted > 0 && malformed > 0);
+}
+
 void serving(std::mt19937 &rng) {
     std::unique_ptr<ArchiveSet> set;
     EXPECT(ArchiveSet::create(set) == Status::success);
@@ -321,6 +406,7 @@ int main() {
     std::mt19937 rng(20261204);
     plans_and_rejections(rng);
     trusted_and_queue(rng);
+    fabricated(rng);
     serving(rng);
     for (void *p : owned()) static_cast<void>(hipFree(p));
     std::printf("failures %d\nrc %d\n", failures, failures == 0 ? 0 : 1);

@@ -190,6 +190,8 @@ PROTOTYPES = {
     "ts_metadata_set_tile": (_u32, [_vp, _u64, _P(TileMetadata)]),
     "ts_metadata_get_segment": (_u32, [_vp, _u64, _P(SegmentMetadata)]),
     "ts_metadata_set_segment": (_u32, [_vp, _u64, _P(SegmentMetadata)]),
+    "ts_metadata_truncate": (_u32, [_vp, _u64, _u64]),
+    "ts_metadata_set_file_bytes": (None, [_vp, _u64]),
     "ts_cpu_compress": (_u32, [ConstImageView, TileShape, MutableBytes, _P(_u64)]),
     "ts_cpu_decompress": (_u32, [ConstBytes, ImageView, _P(ParseLimits)]),
     "ts_tiff_default_options": (None, [_P(TiffReadLimits), _P(TiffWriteOptions)]),

@@ -144,6 +144,13 @@ class Metadata:
         if N.lib().ts_metadata_set_segment(self.handle, i, C.byref(s)):
             raise IndexError(i)
 
+    def truncate(self, tiles: int, segments: int) -> None:
+        if N.lib().ts_metadata_truncate(self.handle, tiles, segments):
+            raise IndexError((tiles, segments))
+
+    def set_file_bytes(self, file_bytes: int) -> None:
+        N.lib().ts_metadata_set_file_bytes(self.handle, file_bytes)
+
 
 def _host_bytes(data) -> tuple:
     if isinstance(data, (bytes, bytearray)):

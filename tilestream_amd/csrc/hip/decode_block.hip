@@ -66,7 +66,7 @@ struct Shared {
     std::uint64_t next[1];
 };
 
-// Requests block i's record into LDS slot `slot` (lanes below 18 of wave 0).
+// Requests block i's record into LDS slot `slot` (lanes below 26 of wave 0).
 __device__ inline void request_record(const tsk_decode_args &a, std::uint64_t i, Shared &sh, std::uint32_t slot) {
     if (threadIdx.x < kRecordBlocks)
         load_to_lds(reinterpret_cast<const u32x4 *>(a.blocks + i) + threadIdx.x, &sh.record[slot][0]);

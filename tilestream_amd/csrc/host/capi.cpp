@@ -214,6 +214,16 @@ uint32_t ts_metadata_set_segment(ts_metadata *m, uint64_t i, const ts_segment_me
     return 0;
 }
 
+uint32_t ts_metadata_truncate(ts_metadata *m, uint64_t tiles, uint64_t segments) {
+    ImageMetadata &x = *meta(m);
+    if (tiles > x.tiles.size() || segments > x.segments.size()) return code(Status::invalid_argument);
+    x.tiles.resize(static_cast<std::size_t>(tiles));
+    x.segments.resize(static_cast<std::size_t>(segments));
+    return 0;
+}
+
+void ts_metadata_set_file_bytes(ts_metadata *m, uint64_t file_bytes) { meta(m)->file_bytes = file_bytes; }
+
 uint32_t ts_cpu_compress(ts_const_image_view input, ts_tile_shape tile, ts_mutable_bytes output, uint64_t *written) {
     return code(cpu::compress(as<ConstImageView>(input), as<TileShape>(tile), as<MutableBytes>(output), *written));
 }
@@ -292,7 +302,8 @@ uint32_t ts_synchronize(void *stream) { return code(hip::synchronize(stream_of(s
 
 uint32_t ts_plan_set_decode_shape(ts_plan *plan, uint32_t group, uint32_t block, uint32_t grid, uint32_t depth) {
     if (plan == nullptr || plan->plan == nullptr) return code(Status::invalid_argument);
-    if (!tsk_decode_supported(group, block, depth)) return code(Status::invalid_argument);
+    // Plans keep tile tables only: the block decoder (group 0) is the queue's.
+    if (group == 0 || !tsk_decode_supported(group, block, depth)) return code(Status::invalid_argument);
     auto *state = const_cast<hip::BatchPlan::Impl *>(plan->plan->impl());
     state->shape = {group, block, grid, depth};
     return 0;
