@@ -42,17 +42,122 @@ def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+def rank_stats(per_call) -> dict:
+    """Nearest-rank p05/p50/p95 (ms) of per-call times."""
+    v = sorted(per_call)
+    rank = lambda p: v[max(0, int(np.ceil(p * len(v))) - 1)]  # noqa: E731
+    return {"p05_ms": round(rank(0.05), 4), "p50_ms": round(rank(0.50), 4), "p95_ms": round(rank(0.95), 4)}
+
+
 def sampled(stream, fn, samples: int = 30, min_ms: float = 2.0):
     """The reference's sample rule (benchmarks/runner.h:197-246) in one process:
     each sample repeats fn until it lasts at least min_ms (at most 1000 calls),
     timed with events on the work's stream; nearest-rank p05/p50/p95 of the
-    per-call times in ms."""
+    per-call times in ms, and the per-call time of every sample."""
     one = timed(stream, fn, 1)
     reps = int(min(1000, max(1, np.ceil(min_ms / max(one, 1e-6)))))
-    per_call = sorted(timed(stream, fn, reps) for _ in range(samples))
-    rank = lambda p: per_call[max(0, int(np.ceil(p * len(per_call))) - 1)]  # noqa: E731
-    return {"samples": samples, "calls_per_sample": reps, "p05_ms": round(rank(0.05), 4),
-            "p50_ms": round(rank(0.50), 4), "p95_ms": round(rank(0.95), 4)}
+    per_call = [timed(stream, fn, reps) for _ in range(samples)]
+    return {"samples": samples, "calls_per_sample": reps, **rank_stats(per_call),
+            "per_call_ms": [round(t, 5) for t in per_call]}
+
+
+def per_image_plans(batch, items, stream):
+    """One decompress plan per image (the reference's per-image method), the
+    workspace they share and one result record each."""
+    from tilestream_amd import api
+    plans, ws_bytes = [], 0
+    for it in items:
+        st, plan = api.prepare_batch(api.Operation.decompress, [it], stream=stream)
+        assert st == api.Status.success, st
+        plans.append(plan)
+        ws_bytes = max(ws_bytes, plan.workspace_bytes())
+    return plans, api.DeviceMemory(ws_bytes), [api.DeviceMemory(16) for _ in plans]
+
+
+def protocol_round(options: dict, shape: tuple, batched_first: bool) -> dict:
+    """One fresh-process round of the sample rule: build the seeded batch,
+    prepare the queue at the given launch shape and the per-image plans, warm
+    up, then sample both methods (in the given order).  Runs in a process
+    forked from a server that never touched the GPU."""
+    from tilestream_amd import api, generators
+    from tilestream_amd.workloads import ResidentBatch
+    layout = generators.batch_layout(options["images"], options["seed"])
+    batch = ResidentBatch.build(layout, options["seed"], options["threads"], keep_pixels=False,
+                                pitch_align=options["pitch_align"])
+    stream = api.Stream()
+    n = len(batch.entries)
+    items = [batch.item(i) for i in range(n)]
+    st, queue = api.DecodeQueue.prepare(items, [api.Operation.decompress] * n, stream)
+    assert st == api.Status.success, st
+    ws = api.DeviceMemory(max(queue.workspace_bytes(), 256))
+    results = api.DeviceMemory(16 * n)
+    assert queue.set_shape(*shape) == api.Status.success
+    assert queue.capture(ws, results, stream) == api.Status.success
+    plans, pws, rec = per_image_plans(batch, items, stream)
+    for plan in plans:
+        assert plan.set_decode_shape(8, 256, 0, 1) == api.Status.success
+
+    def per_image():
+        for plan, r in zip(plans, rec):
+            api.decompress_async(plan, pws, r, 1, stream)
+
+    def batched():
+        queue.launch(stream)
+
+    for _ in range(options["warmup"]):
+        batched()
+        per_image()
+    order = [("batched", batched), ("per_image", per_image)]
+    if not batched_first:
+        order.reverse()
+    out = {name: sampled(stream, fn) for name, fn in order}
+    stream.synchronize()
+    return out
+
+
+def _protocol_child(options, shape, batched_first, conn):
+    try:
+        conn.send(("ok", protocol_round(options, shape, batched_first)))
+    except BaseException as exc:  # reported by the parent
+        conn.send(("error", repr(exc)))
+    conn.close()
+
+
+def protocol_rounds(ctx, rounds: int, options: dict, shape: tuple, timeout_s: float = 240.0) -> dict:
+    """The reference's timing protocol (docs/benchmarking.md "Rounds"): the
+    sample rule in `rounds` separate processes, the order of the two methods
+    alternating between rounds, all samples merged (nearest-rank statistics
+    over every sample) with each round's median kept."""
+    merged = {"batched": [], "per_image": []}
+    medians = {"batched": [], "per_image": []}
+    reps = {"batched": [], "per_image": []}
+    for r in range(rounds):
+        parent, child = ctx.Pipe(duplex=False)
+        proc = ctx.Process(target=_protocol_child, args=(options, shape, r % 2 == 0, child))
+        proc.start()
+        child.close()
+        if not parent.poll(timeout_s):
+            proc.kill()
+            proc.join()
+            raise RuntimeError(f"protocol round {r + 1} sent nothing within {timeout_s:.0f} s")
+        try:
+            kind, payload = parent.recv()
+        except EOFError:
+            proc.join(60)
+            raise RuntimeError(f"protocol round {r + 1} ended with exit code {proc.exitcode} and no result")
+        proc.join(60)
+        if kind != "ok":
+            raise RuntimeError(f"protocol round {r + 1} failed: {payload}")
+        for name in merged:
+            merged[name] += payload[name]["per_call_ms"]
+            medians[name].append(payload[name]["p50_ms"])
+            reps[name].append(payload[name]["calls_per_sample"])
+        log(f"protocol round {r + 1}/{rounds}: batched p50 {payload['batched']['p50_ms']} ms, "
+            f"per-image p50 {payload['per_image']['p50_ms']} ms")
+    out = {name: {"samples": len(merged[name]), "calls_per_sample": min(reps[name]), **rank_stats(merged[name]),
+                  "round_p50_ms": medians[name]} for name in merged}
+    out["rounds"] = rounds
+    return out
 
 
 def timed(stream, fn, steps: int):
@@ -65,6 +170,16 @@ def timed(stream, fn, steps: int):
     stop.record(stream)
     stop.synchronize()
     return start.elapsed_ms(stop) / steps
+
+
+def host_cores() -> dict:
+    """The host's logical CPU count and the CPUs this process may run on (the
+    baseline's thread count is min(16, the latter))."""
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        usable = os.cpu_count() or 1
+    return {"host_logical_cpus": os.cpu_count(), "host_cpus_usable": usable}
 
 
 def cpu_baseline(batch, threads: int, single_budget_bytes: int):
@@ -96,7 +211,7 @@ def cpu_baseline(batch, threads: int, single_budget_bytes: int):
     single_bytes = sum(decode(e) for e in sample)
     single = time.perf_counter() - t0
     return {
-        "value": round(total / multi / 1e9, 3), "unit": "GB/s", "cores": threads, "kind": kind,
+        "value": round(total / multi / 1e9, 3), "unit": "GB/s", "cores": threads, "kind": kind, **host_cores(),
         "sample": f"whole-image decode of all {len(entries)} archives of the batch "
                   f"({total / 1e9:.2f} GB decoded) on {threads} threads, one image per thread",
         "single_thread": {"value": round(single_bytes / single / 1e9, 3), "unit": "GB/s", "cores": 1,
@@ -188,8 +303,9 @@ def roi_leg(args, threads: int, stream):
         f"{decoded / 1e9:.2f} GB decoded, {total / 1e9:.2f} GB FP32 out, set up in {time.perf_counter() - t0:.1f} s")
 
     shape = tuple(int(v) for v in args.roi_shape.split(","))
-    # One server per configured stream count: fewer streams keep tiny batches
-    # from sharing the GPU at low load, more overlap them at high load.
+    # One server per configured stream count.  The first count is the headline
+    # configuration; it is fixed by the command line, never chosen from the
+    # results.
     stream_counts = [int(v) for v in str(args.roi_streams).split(",")]
     servers = {k: api.Server(aset, api.server_options(streams=k, max_batch=args.roi_max_batch, shape=shape))
                for k in stream_counts}
@@ -219,10 +335,11 @@ def roi_leg(args, threads: int, stream):
             log(f"roi latency at {q} req/s, {k} streams: p50 {rec['p50_us']} us, p99 {rec['p99_us']} us, "
                 f"{stats.batches} batches")
             by_streams.append(rec)
-        best = dict(min(by_streams, key=lambda r: r["p99_us"]))
-        best["by_streams"] = [{kk: r[kk] for kk in ("streams", "p50_us", "p99_us", "max_us", "batches")}
-                              for r in by_streams]
-        latency.append(best)
+        head = dict(by_streams[0])
+        if len(by_streams) > 1:
+            head["other_stream_counts"] = [{kk: r[kk] for kk in ("streams", "p50_us", "p99_us", "max_us", "batches")}
+                                           for r in by_streams[1:]]
+        latency.append(head)
 
     # The same requests into outputs whose column 0 sits at the 128-byte phase
     # of its archive column in a full-frame FP32 buffer, so that tile
@@ -318,9 +435,8 @@ def roi_leg(args, threads: int, stream):
         "per_request_path": plain,
         "phase_aligned_outputs": phased_rec,
         "server": {"streams": stream_counts, "max_batch": args.roi_max_batch, "shape": list(shape),
-                   "note": "saturation, phase-aligned and verification replays use the first stream count; "
-                           "each latency record is the stream count with the lowest p99 at that rate, "
-                           "every count listed under by_streams (same arrivals)"},
+                   "note": f"every headline figure uses {stream_counts[0]} streams, fixed before the run; "
+                           "other counts (if any) are listed under other_stream_counts, same arrivals"},
         "verified_bit_exact": verified,
     }
 
@@ -352,7 +468,7 @@ def encode_cpu_baseline(batch, threads: int, sample_bytes: int):
     with cf.ThreadPoolExecutor(threads) as pool:
         same = all(pool.map(one, sample)) and same
     multi = time.perf_counter() - t0
-    return {"value": round(acc / multi / 1e9, 3), "unit": "GB/s of pixels", "cores": threads,
+    return {"value": round(acc / multi / 1e9, 3), "unit": "GB/s of pixels", "cores": threads, **host_cores(),
             "kind": "reference" if ref else "port", "single_thread_GBps": round(acc / single / 1e9, 3),
             "identical_archives": same,
             "sample": f"first {len(sample)} images of the batch ({acc / 1e9:.2f} GB of pixels), one image per thread"}
@@ -545,53 +661,92 @@ def serve_mix_leg(args, threads: int, stream):
             f"encode p99 {rec['encode']['p99_us']} us")
         latency.append(rec)
 
+    # The same requests under an admission cap that binds: an eighth of the
+    # replay's output bytes (never below the largest single request).
+    sizes = [size for *_, size, _ in specs]
+    cap = max(max(sizes), (out_total + enc_total) // 8)
+    bounded = api.Server(aset, api.server_options(streams=args.mix_streams, max_batch=args.roi_max_batch,
+                                                  admission_bytes=cap, shape=shape))
+    for _ in range(2):
+        st, _, statuses, _, bsat = bounded.replay(requests, np.zeros(n, np.uint64))
+        assert st == api.Status.success and (statuses == 0).all(), st
+    arrivals = np.cumsum(np.random.default_rng(args.seed + 23).exponential(1e9 / args.mix_qps[0], n)).astype(np.uint64)
+    st, blat, statuses, _, bstats = bounded.replay(requests, arrivals)
+    assert st == api.Status.success and (statuses == 0).all(), st
+    admission = {"admission_bytes": int(cap), "saturation_ms": round(bsat.seconds * 1e3, 3),
+                 "requests_per_s": round(n / bsat.seconds, 1), "GBps": round(bsat.decoded_bytes / bsat.seconds / 1e9, 2),
+                 "batches": int(bsat.batches), "admission_holds": int(bsat.admission_holds),
+                 "peak_in_flight_bytes": int(bsat.peak_in_flight),
+                 "qps": args.mix_qps[0], "p50_us": round(percentile(blat, 50) / 1e3, 1),
+                 "p99_us": round(percentile(blat, 99) / 1e3, 1), "holds_at_qps": int(bstats.admission_holds),
+                 "note": "admission cap = max(largest request, 1/8 of the replay's output bytes); saturation "
+                         "and Poisson replays of the same requests"}
+    log(f"serve mix with a {cap / 1e9:.2f} GB admission cap: {admission['saturation_ms']} ms saturated, "
+        f"{bsat.admission_holds} holds, peak {bsat.peak_in_flight / 1e9:.2f} GB in flight; p99 "
+        f"{admission['p99_us']} us at {args.mix_qps[0]:.0f} req/s")
+
     verified = None
     if not args.no_verify:
-        # Replay into cleared outputs; check every encode's size, a sample of
-        # encodes byte for byte against the host codec's archive, and a sample
-        # of decodes against the generated pixels.
-        out.fill(0)
-        enc.fill(0)
-        st, _, statuses, written, _ = server.replay(requests, np.zeros(n, np.uint64))
-        ok = st == api.Status.success and bool((statuses == 0).all())
-        checked = {0: 0, 1: 0, 2: 0}
-        limit = {0: 200, 1: 12, 2: 12}
-        for i, (k, a, roi, off, size, pitch) in enumerate(specs):
-            e = rb.entries[a]
-            if k == 2:
-                ok = ok and int(written[i]) == len(e.archive)
-                if checked[2] < limit[2]:
-                    ok = ok and enc.download(len(e.archive), off).tobytes() == e.archive
-                    checked[2] += 1
-                continue
-            if checked[k] >= limit[k]:
-                continue
-            x, y, w, h = roi
-            buf = np.zeros(pitch * h, np.uint8)
-            buf[:size] = out.download(size, off)
-            got = np.ascontiguousarray(buf.reshape(h, pitch)[:, :w * e.sample]).view(e.pixels.dtype)
-            ok = ok and np.array_equal(got, e.pixels[y:y + h, x:x + w])
-            checked[k] += 1
+        # Replay into cleared outputs (once without and once with the binding
+        # cap); check every encode's size, a sample of encodes byte for byte
+        # against the host codec's archive, and a sample of decodes against
+        # the generated pixels.
+        ok = True
+        for srv in (server, bounded):
+            ok = ok and verify_mix(srv, requests, specs, rb, out, enc)
         verified = bool(ok)
-        log(f"serve mix verification bit-exact: {verified} (checked {checked})")
+        log(f"serve mix verification bit-exact: {verified}")
     return {
         "workload": f"{n} requests (70% region decode, sides log-uniform in [32, 2048]; 20% whole decode; "
                     f"10% GPU encode) over {len(rb.entries)} images with sides from {{512..8192}}, 50/50 "
                     f"uint16/float32, tiles from {{128x32, 64x64, 256x16}}; Poisson arrivals; admission cap "
-                    f"200 GB of output bytes in flight",
+                    f"200 GB of output bytes in flight (never binding); admission_bound: the same with a cap "
+                    f"that binds",
         "requests": {"region": counts[0], "whole": counts[1], "encode": counts[2]},
         "images_raw_bytes": rb.decoded_bytes,
         "saturation": {"ms": round(sat.seconds * 1e3, 3), "requests_per_s": round(n / sat.seconds, 1),
                        "GBps": round(sat.decoded_bytes / sat.seconds / 1e9, 2), "batches": int(sat.batches),
-                       "bytes": int(sat.decoded_bytes),
+                       "bytes": int(sat.decoded_bytes), "admission_holds": int(sat.admission_holds),
                        "note": "bytes = pixels decoded (stored sample size) + pixels encoded"},
         "latency": latency,
+        "admission_bound": admission,
         "server": {"streams": args.mix_streams, "max_batch": args.roi_max_batch, "shape": list(shape),
                    "large_bytes": int(opts.large_bytes),
                    "note": "requests with at least large_bytes of output batch apart, on streams - 1 at most, "
                            "with (streams - 1) / streams of the decode grid"},
         "verified_bit_exact": verified,
     }
+
+
+def verify_mix(server, requests, specs, rb, out, enc) -> bool:
+    """One saturation replay of the mix into cleared outputs, checked against
+    the generated pixels and the host codec's archives (a sample of each kind)."""
+    from tilestream_amd import api
+    n = len(requests)
+    out.fill(0)
+    enc.fill(0)
+    st, _, statuses, written, _ = server.replay(requests, np.zeros(n, np.uint64))
+    ok = st == api.Status.success and bool((statuses == 0).all())
+    checked = {0: 0, 1: 0, 2: 0}
+    limit = {0: 200, 1: 12, 2: 12}
+    for i, (k, a, roi, off, size, pitch) in enumerate(specs):
+        e = rb.entries[a]
+        if k == 2:
+            ok = ok and int(written[i]) == len(e.archive)
+            if checked[2] < limit[2]:
+                ok = ok and enc.download(len(e.archive), off).tobytes() == e.archive
+                checked[2] += 1
+            continue
+        if checked[k] >= limit[k]:
+            continue
+        x, y, w, h = roi
+        buf = np.zeros(pitch * h, np.uint8)
+        buf[:size] = out.download(size, off)
+        got = np.ascontiguousarray(buf.reshape(h, pitch)[:, :w * e.sample]).view(e.pixels.dtype)
+        ok = ok and np.array_equal(got, e.pixels[y:y + h, x:x + w])
+        checked[k] += 1
+    log(f"serve mix check (admission {server.options.admission_bytes / 1e9:.2f} GB): {ok} (checked {checked})")
+    return bool(ok)
 
 
 def staged_leg(args, threads: int, stream):
@@ -1090,6 +1245,8 @@ def fused_leg(args, stream):
 def main() -> int:
     p = argparse.ArgumentParser()
     p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--protocol-rounds", type=int, default=5,
+                   help="fresh processes for the batched vs per-image sample rule (0: sample in this process)")
     p.add_argument("--steps", type=int, default=20)
     p.add_argument("--warmup", type=int, default=5)
     p.add_argument("--images", type=int, default=256)
@@ -1112,9 +1269,10 @@ def main() -> int:
     p.add_argument("--roi-only", action="store_true", help="only the ROI serving leg")
     p.add_argument("--roi-archives", type=int, default=64)
     p.add_argument("--roi-requests", type=int, default=10000)
-    p.add_argument("--roi-streams", default="2,3",
-                   help="server stream counts for the ROI leg's latency replays (comma-separated; the first "
-                        "also serves the saturation and verification replays)")
+    p.add_argument("--roi-streams", default="3",
+                   help="server stream counts for the ROI leg (comma-separated).  The first is fixed before "
+                        "the run and gives every headline figure (saturation, p50/p99 at each rate, "
+                        "verification); further counts are replayed on the same arrivals for comparison only")
     p.add_argument("--mix-streams", type=int, default=3,
                    help="server streams for the serving-mix and staged legs (3 gave the most throughput)")
     p.add_argument("--roi-max-batch", type=int, default=1024)
@@ -1147,6 +1305,17 @@ def main() -> int:
               "multi-GPU path: shard images or requests across processes, one per GPU)", file=sys.stderr)
         return 2
 
+    forkserver = None
+    only = (args.file_only or args.staged_only or args.mix_only or args.fused_only or args.roi_only
+            or args.encode_only or args.profile or args.copy_probe)
+    if args.protocol_rounds > 0 and not only:
+        # The protocol's fresh processes fork from this server, started before
+        # anything here touches the GPU.
+        import multiprocessing as mp
+        from multiprocessing import forkserver as mp_forkserver
+        forkserver = mp.get_context("forkserver")
+        mp_forkserver.ensure_running()
+
     from tilestream_amd import api, generators
     from tilestream_amd.workloads import ResidentBatch
 
@@ -1163,7 +1332,7 @@ def main() -> int:
         s.synchronize()
         log(f"copy probe: {args.steps} copies of {nbytes} bytes (read {nbytes}, write {nbytes} each)")
         return 0
-    threads = max(1, min(16, os.cpu_count() or 1))
+    threads = max(1, min(16, host_cores()["host_cpus_usable"]))
     if args.file_only:
         fl = file_leg(args, api.Stream())
         print(json.dumps({"metric": "file in page cache to calibrated FP32", "file": fl}), flush=True)
@@ -1261,16 +1430,8 @@ def main() -> int:
     # The reference's per-image method: one launch per image, one workgroup per
     # tile.  The plans' own default (a persistent stride grid for plain decodes
     # of 2048 tiles or more) is timed beside it.
-    plans, default_shapes, ws_bytes = [], [], 0
-    for i in range(n):
-        st, plan = api.prepare_batch(api.Operation.decompress, [items[i]], stream=stream)
-        assert st == api.Status.success, st
-        plans.append(plan)
-        e = batch.entries[i]
-        default_shapes.append(-(-e.width // e.tile[0]) * -(-e.height // e.tile[1]) >= 2048)
-        ws_bytes = max(ws_bytes, plan.workspace_bytes())
-    pws = api.DeviceMemory(ws_bytes)
-    rec = [api.DeviceMemory(16) for _ in range(n)]
+    plans, pws, rec = per_image_plans(batch, items, stream)
+    default_shapes = [-(-e.width // e.tile[0]) * -(-e.height // e.tile[1]) >= 2048 for e in batch.entries]
 
     def per_image():
         for plan, r in zip(plans, rec):
@@ -1287,10 +1448,19 @@ def main() -> int:
     for _ in range(args.warmup):
         per_image()
     ms_plain = timed(stream, per_image, max(3, args.steps // 4))
-    protocol = {"batched": sampled(stream, launch), "per_image": sampled(stream, per_image),
-                "note": "reference sample rule: 30 samples of at least 2 ms, events on the stream, nearest-rank "
-                        "percentiles of the per-launch time (one process; the reference also repeats the "
-                        "process 5 times)"}
+    if args.protocol_rounds > 0:
+        opts = {"images": args.images, "seed": args.seed, "threads": threads, "pitch_align": args.pitch_align,
+                "warmup": args.warmup}
+        shape = (best.group, best.block, best.grid, best.chunk, best.depth, best.run)
+        protocol = protocol_rounds(forkserver, args.protocol_rounds, opts, shape)
+        protocol["note"] = ("reference timing protocol: the sample rule (30 samples of at least 2 ms, events on "
+                            "the stream) in fresh processes, alternating the order of the two methods, samples "
+                            "of every round merged, nearest-rank percentiles of the per-launch time")
+    else:
+        protocol = {"batched": sampled(stream, launch), "per_image": sampled(stream, per_image),
+                    "note": "reference sample rule in this process only (30 samples of at least 2 ms)"}
+        for name in ("batched", "per_image"):
+            protocol[name].pop("per_call_ms")
     protocol["speedup_p50"] = round(protocol["per_image"]["p50_ms"] / protocol["batched"]["p50_ms"], 3)
     log(f"protocol: batched p50 {protocol['batched']['p50_ms']} ms, per-image p50 {protocol['per_image']['p50_ms']} "
         f"ms, speedup {protocol['speedup_p50']}")
@@ -1383,7 +1553,8 @@ def main() -> int:
     }
     if roi is not None:
         out["p99_roi_latency_us"] = {str(int(r["qps"])): r["p99_us"] for r in roi["latency"]}
-        out["p99_roi_latency_streams"] = {str(int(r["qps"])): r["streams"] for r in roi["latency"]}
+        out["p50_roi_latency_us"] = {str(int(r["qps"])): r["p50_us"] for r in roi["latency"]}
+        out["roi_streams"] = roi["latency"][0]["streams"] if roi["latency"] else None
     print(json.dumps(out), flush=True)
     ok = verified in (None, True) and all(leg is None or leg["verified_bit_exact"] in (None, True)
                                           for leg in (roi, mix, enc, staged, fl))

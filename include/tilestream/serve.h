@@ -101,6 +101,8 @@ struct ServerStats {
     std::uint64_t decoded_bytes = 0;     // region pixels x stored sample bytes, plus encoded input bytes
     std::uint64_t failed = 0;            // requests whose device result was not success
     double seconds = 0.0;                // first arrival to last completion
+    std::uint64_t admission_holds = 0;   // batches closed early because admission_bytes was reached
+    std::uint64_t peak_in_flight = 0;    // most output bytes admitted and not yet finished
 };
 
 class Server {

@@ -214,7 +214,12 @@ typedef struct {
     ts_launch_shape shape;
     uint64_t large_bytes;          /* requests with this many output bytes batch apart, on streams - 1 at most */
 } ts_server_options;
-typedef struct { uint64_t requests, batches, decoded_bytes, failed; double seconds; } ts_server_stats;
+typedef struct {
+    uint64_t requests, batches, decoded_bytes, failed;
+    double seconds;
+    uint64_t admission_holds; /* batches closed early because admission_bytes was reached */
+    uint64_t peak_in_flight;  /* most output bytes admitted and not yet finished */
+} ts_server_stats;
 uint32_t ts_archive_set_create(ts_archive_set **set);
 void ts_archive_set_destroy(ts_archive_set *set);
 /* `metadata` is the parse of the same bytes; the bytes stay resident and unchanged. */

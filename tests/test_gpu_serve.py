@@ -171,8 +171,38 @@ def test_admission_smaller_than_one_request_still_progresses(orc, resident):
     st, latency, statuses, _, stats = server.replay(out.requests, np.zeros(len(specs), np.uint64))
     assert st == ts.Status.success and (statuses == 0).all()
     assert stats.batches >= len(specs) // 2  # the large regions go one at a time
+    assert stats.admission_holds > 0
+    # Requests above the cap run alone: the peak is one request's output.
+    assert stats.peak_in_flight <= max(_output_bytes(resident, specs))
     for i, spec in enumerate(specs):
         assert _same(out.read(specs, i), _expected(orc, resident, spec))
+
+
+def _output_bytes(resident, specs):
+    return [w * h * (4 if cal or resident.pixels[a].dtype != np.uint16 else 2) for a, (x, y, w, h), cal in specs]
+
+
+@pytest.mark.parametrize("streams", [1, 3])
+def test_admission_cap_binds_and_bounds_bytes_in_flight(orc, resident, streams):
+    """A cap well below the outputs of the whole replay but above any single
+    request: batches close early at the cap, the output bytes in flight never
+    exceed it, and every result still equals the oracle."""
+    rng = np.random.default_rng(29 + streams)
+    specs = _requests(resident, rng, 80, big_every=3)
+    sizes = _output_bytes(resident, specs)
+    cap = max(max(sizes), sum(sizes) // 6)
+    out = Outputs(resident, specs)
+    server = api.Server(resident.set, api.server_options(streams=streams, admission_bytes=cap))
+    st, latency, statuses, _, stats = server.replay(out.requests, np.zeros(len(specs), np.uint64))
+    assert st == ts.Status.success and (statuses == 0).all()
+    assert stats.admission_holds > 0
+    assert 0 < stats.peak_in_flight <= cap
+    for i, spec in enumerate(specs):
+        assert _same(out.read(specs, i), _expected(orc, resident, spec)), (streams, i, spec)
+    # Without a binding cap the same replay never holds.
+    free = api.Server(resident.set, api.server_options(streams=streams, admission_bytes=sum(sizes) + 1))
+    st, _, statuses, _, stats = free.replay(out.requests, np.zeros(len(specs), np.uint64))
+    assert st == ts.Status.success and (statuses == 0).all() and stats.admission_holds == 0
 
 
 def test_request_checks(resident):

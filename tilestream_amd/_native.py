@@ -157,7 +157,8 @@ class TiffWriteOptions(C.Structure):
 
 class ServerStats(C.Structure):
     _fields_ = [("requests", C.c_uint64), ("batches", C.c_uint64), ("decoded_bytes", C.c_uint64),
-                ("failed", C.c_uint64), ("seconds", C.c_double)]
+                ("failed", C.c_uint64), ("seconds", C.c_double), ("admission_holds", C.c_uint64),
+                ("peak_in_flight", C.c_uint64)]
 
 
 RECORDS = {

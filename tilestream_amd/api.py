@@ -658,6 +658,7 @@ class Server:
             raise RuntimeError(f"ts_server_create: {st.name}")
         self.handle = handle.value
         self._archives = archives
+        self.options = opts
 
     def check(self, request: ServeRequest) -> Status:
         return Status(N.lib().ts_server_check(self.handle, C.byref(request)))

@@ -1,7 +1,8 @@
 // Standalone calibration y = RN(RN(x - offset) * gain) for gfx950
 // (reference src/cuda/calibrate.cu:13-62; docs/numerical-analysis.md).
-// Rows of all images are concatenated; a workgroup takes rows with a stride and
-// its lanes stream the row four pixels at a time.
+// Rows of all images are concatenated; a workgroup takes rows with a stride,
+// finds the row's image by binary search over first_rows, and its lanes stream
+// the row one pixel per lane per step (consecutive lanes, consecutive pixels).
 #include <hip/hip_runtime.h>
 
 #include <tilestream_kernels.h>

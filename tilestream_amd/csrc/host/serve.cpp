@@ -713,11 +713,15 @@ Status Server::replay(const ServeRequest *requests, const std::uint64_t *arrival
                 const std::uint64_t *ids = queue[q].data() + head[q];
                 const std::uint64_t waiting = queue[q].size() - head[q];
                 std::uint64_t n = 0, tasks = 0, bytes = 0, staged = 0;
+                bool held = false;
                 while (n < waiting && n < s.options.max_batch) {
                     const Extent &e = s.extents[ids[n]];
                     if (tasks + e.tasks > s.options.max_tasks) break;
                     if (staged + e.staged > s.options.staging_bytes) break;
-                    if (in_flight + bytes + e.bytes > s.options.admission_bytes && (n != 0 || in_flight != 0)) break;
+                    if (in_flight + bytes + e.bytes > s.options.admission_bytes && (n != 0 || in_flight != 0)) {
+                        held = true;
+                        break;
+                    }
                     tasks += e.tasks;
                     bytes += e.bytes;
                     staged += e.staged;
@@ -732,6 +736,8 @@ Status Server::replay(const ServeRequest *requests, const std::uint64_t *arrival
                 slot.large = q == 1;
                 if (slot.large) ++large_busy;
                 in_flight += bytes;
+                stats.peak_in_flight = std::max(stats.peak_in_flight, in_flight);
+                if (held) ++stats.admission_holds;
                 head[q] += n;
                 dispatched += n;
                 ++stats.batches;
