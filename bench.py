@@ -249,6 +249,31 @@ def copy_ceiling(stream, total_bytes: int, steps: int = 5):
     return 2 * half / (ms * 1e-3) / 1e9
 
 
+WARM_MS = 0.0  # --warm-ms: settle() before each measured section
+
+
+def settle() -> None:
+    if WARM_MS > 0:
+        gpu_warm(WARM_MS)
+
+
+def gpu_warm(ms: float) -> None:
+    """Runs a plain streaming copy (1 GiB each way) for `ms` of wall time so
+    that clocks and power settle before anything is measured or profiled: on a
+    cold GPU the first launches of a kernel run up to 12% slower for about 10
+    launches (clock and power transient)."""
+    from tilestream_amd import api
+    from tilestream_amd import _native as N
+    size = 1 << 30
+    a, b, s = api.DeviceMemory(size), api.DeviceMemory(size), api.Stream()
+    a.fill(0)
+    end = time.perf_counter() + ms * 1e-3
+    while time.perf_counter() < end:
+        for _ in range(8):
+            assert N.lib().tsk_stream_copy(b.ptr, a.ptr, size // 16, s.handle) == 0
+        s.synchronize()
+
+
 def percentile(values, q: float) -> float:
     """Nearest-rank percentile (the reference harness's convention)."""
     v = np.sort(np.asarray(values))
@@ -313,6 +338,7 @@ def roi_leg(args, threads: int, stream):
     n = len(requests)
     # Saturation: every request arrives at once; continuous batching drains them.
     sat = None
+    settle()
     for _ in range(2):
         st, lat, statuses, _, stats = server.replay(requests, np.zeros(n, np.uint64))
         assert st == api.Status.success and (statuses == 0).all(), st
@@ -523,6 +549,7 @@ def encode_leg(args, batch, stream):
         for plan, res, members in plans:
             api.compress_async(plan, ws, res, len(members), stream)
 
+    settle()
     for _ in range(args.warmup):
         step()
     ms = timed(stream, step, args.steps)
@@ -815,6 +842,7 @@ def staged_leg(args, threads: int, stream):
     server = api.Server(aset, api.server_options(streams=args.mix_streams, max_batch=args.roi_max_batch,
                                                  staging_bytes=args.staging_mib << 20, shape=shape))
     sat = None
+    settle()
     for _ in range(2):
         st, lat, statuses, _, stats = server.replay(requests, np.zeros(n, np.uint64))
         assert st == api.Status.success and (statuses == 0).all(), st
@@ -1177,6 +1205,7 @@ def fused_leg(args, stream):
             api.decode_calibrate_async(plans[step[0] % sets], ws, res, 1, stream)
             step[0] += 1
 
+        settle()
         for _ in range(args.warmup):
             run()
         ms = timed(stream, run, args.steps)
@@ -1222,6 +1251,7 @@ def fused_leg(args, stream):
             api.calibrate_async(cal_plans[step2[0] % sets], ws2, res, 1, stream)
             step2[0] += 1
 
+        settle()
         for _ in range(args.warmup):
             run_cal()
         cal_ms = timed(stream, run_cal, args.steps)
@@ -1258,6 +1288,8 @@ def main() -> int:
                    help="only the batched launches (for rocprofv3): no autotune, per-image path, "
                         "CPU baseline or verification")
     p.add_argument("--shape", default="8,256,8,2", help="group,block,chunk,depth used with --profile")
+    p.add_argument("--warm-ms", type=float, default=0.0,
+                   help="run a streaming copy for this long before anything else (settles clocks for profiles)")
     p.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                    help="only run a plain copy kernel of this many MiB (scales HBM byte counters)")
     p.add_argument("--pitch-align", type=int, default=256,
@@ -1322,6 +1354,8 @@ def main() -> int:
     if api.device_count() < 1:
         print("no GPU visible", file=sys.stderr)
         return 1
+    global WARM_MS
+    WARM_MS = args.warm_ms
     if args.copy_probe:
         from tilestream_amd import _native as N
         nbytes = args.copy_probe << 20
@@ -1390,6 +1424,7 @@ def main() -> int:
     log(f"shape group={best.group} block={best.block} chunk={best.chunk} depth={best.depth}")
     assert queue.capture(qws, results, stream) == api.Status.success
     launch = lambda: queue.launch(stream)  # noqa: E731
+    settle()
     for _ in range(args.warmup):
         launch()
     ms_batched = timed(stream, launch, args.steps)
@@ -1397,6 +1432,11 @@ def main() -> int:
     if args.profile:
         log(f"batched {ms_batched:.4f} ms per launch, "
             f"{batch.decoded_bytes / (ms_batched * 1e-3) / 1e9:.1f} GB/s decoded")
+        print(json.dumps({"metric": "batched decode launches for a profiler", "ms_per_launch": round(ms_batched, 4),
+                          "steps": args.steps, "warmup": args.warmup, "warm_ms": args.warm_ms,
+                          "shape": [best.group, best.block, best.grid, best.chunk, best.depth, best.run],
+                          "tiles": int(stats.tasks), "payload_bytes": int(stats.payload_bytes),
+                          "output_bytes": int(stats.output_bytes), "map_bytes": int(stats.map_bytes)}), flush=True)
         return 0
 
     # ---- the same launch into packed output rows (pitch = width x sample) -----------

@@ -44,6 +44,21 @@ class ArchiveSet {
     // into `metadata` (parse_metadata of the same bytes).  The bytes must stay
     // allocated and unchanged while the set is in use.
     [[nodiscard]] Status add(ConstBytes device_file, const ImageMetadata &metadata, std::uint32_t &id) noexcept;
+    // Adds an archive in device memory of the current device without host
+    // metadata: its header, index and payloads are validated on the device,
+    // with the host parser's rules and header statuses, and its tile table is
+    // built there (no copy of the archive to the host).  `image` and `tile`
+    // are what the caller expects the file to hold (a valid file of another
+    // image is invalid_argument).  With `producer` (the DeviceResult of the
+    // encode that wrote the file, in device memory), the file size is its
+    // bytes_written and a failed encode's status is returned; without it the
+    // file is exactly device_file.size bytes.  Work is ordered on `stream`
+    // after whatever wrote the file, and the call waits for it.  No reference
+    // counterpart: the reference's trusted decode checks encoder output per
+    // decode (src/cuda/decoder.cu:86-110,191-220); here it is checked once
+    // and served many times.
+    [[nodiscard]] Status add_device(ConstBytes device_file, const ImageSpec &image, TileShape tile,
+                                    const DeviceResult *producer, hipStream_t stream, std::uint32_t &id) noexcept;
     // Uploads the tables of every archive added so far (synchronizes `stream`).
     [[nodiscard]] Status commit(hipStream_t stream) noexcept;
 

@@ -225,6 +225,11 @@ void ts_archive_set_destroy(ts_archive_set *set);
 /* `metadata` is the parse of the same bytes; the bytes stay resident and unchanged. */
 uint32_t ts_archive_set_add(ts_archive_set *set, ts_const_bytes device_file, const ts_metadata *metadata,
                             uint32_t *id);
+/* An archive in device memory without host metadata: validated and tabled on
+ * the device (tilestream::hip::ArchiveSet::add_device; no reference
+ * counterpart).  `producer` (may be NULL) is the encode's device result. */
+uint32_t ts_archive_set_add_device(ts_archive_set *set, ts_const_bytes device_file, ts_image_spec image,
+                                   ts_tile_shape tile, const ts_device_result *producer, void *stream, uint32_t *id);
 uint32_t ts_archive_set_commit(ts_archive_set *set, void *stream);
 uint32_t ts_archive_set_size(const ts_archive_set *set);
 uint32_t ts_archive_set_info(const ts_archive_set *set, uint32_t id, ts_archive_info *out);

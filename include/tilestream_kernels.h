@@ -194,7 +194,10 @@ typedef struct {
     uint32_t tile_count;         /* tiles in this window */
     uint32_t parity;             /* window number & 1 */
     uint32_t type;               /* TSK_UINT16 / TSK_FLOAT32 */
-    uint32_t reserved;
+    /* Consecutive window tiles per XCD: workgroups b with one b % 8 run on one
+     * XCD, so runs of neighbouring tiles share that XCD's L2 (rows of
+     * neighbouring tiles share cache lines).  0 or 1: workgroup b, tile b. */
+    uint32_t run;
 } tsk_encode_args;
 
 /* Per-image descriptor of standalone calibration. */
@@ -216,6 +219,40 @@ typedef struct {
     uint32_t image_count;
     uint32_t type;
 } tsk_calibrate_args;
+
+/* Device-built resident tile table of one archive in device memory. */
+typedef struct {
+    uint32_t header_status;      /* 0, a parser status, or TSK_INDEX_OTHER_IMAGE */
+    uint32_t tile_failures;      /* tiles whose entry or payload failed a check */
+    uint64_t file_bytes;         /* the file's size when the header is valid */
+} tsk_index_record;
+
+/* The header is valid but describes another image than the caller's. */
+#define TSK_INDEX_OTHER_IMAGE 1u
+
+typedef struct {
+    const uint8_t *src;          /* the .tsc file, 4-byte aligned */
+    const void *producer;        /* the encoder's DeviceResult (status, file size), or NULL */
+    tsk_decode_task *tasks;      /* out: tile_count request-mode task entries (image 0) */
+    uint16_t *rows;              /* out: tile_h row-descriptor word offsets per tile, at rows + i x tile_h */
+    tsk_index_record *record;    /* out */
+    uint64_t capacity;           /* readable bytes at src; the file size when producer is NULL */
+    uint64_t bound;              /* compress bound of the expected image and tile shape */
+    uint64_t tile_count;         /* tiles of the expected image */
+    uint32_t width, height;      /* expected image */
+    uint32_t tile_columns;
+    uint32_t rows_base;          /* row-table index of rows[0] in the resident table */
+    uint16_t tile_w, tile_h;
+    uint32_t type;               /* TSK_UINT16 / TSK_FLOAT32 */
+} tsk_index_args;
+
+/* Validates a .tsc file in device memory as the host parser does and writes
+ * its request-mode tile table.  tsk_index_header: one thread checks the
+ * header and writes the record (tile_failures zeroed; tasks and rows unused).
+ * tsk_index_tiles: one thread per tile checks its entry and payload and
+ * writes its table entry, when the record's header status is 0. */
+int tsk_index_header(const tsk_index_args *args, void *stream);
+int tsk_index_tiles(const tsk_index_args *args, void *stream);
 
 /* Zeroes `count` result records and, when `counter` is not NULL, `words`
  * (at most 256) work-counter words, in one kernel. */

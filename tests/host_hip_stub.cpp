@@ -341,6 +341,22 @@ int tsk_encode_window(const tsk_encode_args *a, void *) {
     return hipSuccess;
 }
 
+int tsk_index_header(const tsk_index_args *a, void *) {
+    check(inside_allocation(a->record, sizeof(tsk_index_record)), "index record");
+    check(inside_allocation(a->src, a->capacity), "index source");
+    std::uint64_t size = a->capacity;
+    if (a->producer != nullptr) std::memcpy(&size, static_cast<const std::uint8_t *>(a->producer) + 8, 8);
+    *a->record = tsk_index_record{0u, 0u, size};
+    return hipSuccess;
+}
+
+int tsk_index_tiles(const tsk_index_args *a, void *) {
+    check(inside_allocation(a->tasks, a->tile_count * sizeof(tsk_decode_task)), "index tasks");
+    check(inside_allocation(a->rows, a->tile_count * a->tile_h * sizeof(std::uint16_t)), "index rows");
+    std::memset(a->tasks, 0, a->tile_count * sizeof(tsk_decode_task));
+    return hipSuccess;
+}
+
 int tsk_scratch_region(void **address, uint64_t *bytes) {
     static void *scratch = allocate(TSK_SCRATCH_BYTES, Kind::device);
     *address = scratch;

@@ -357,6 +357,25 @@ void serving(std::mt19937 &rng) {
         EXPECT(set->add(ConstBytes{archives.back().device, archives.back().file.size()}, archives.back().meta, id) ==
                Status::success);
     }
+    // The same archives again, tabled on the device (the stand-in launchers
+    // check the table spans), before and after a commit.
+    for (std::size_t i = 0; i < archives.size(); ++i) {
+        const Archive &a = archives[i];
+        std::uint32_t id = 0;
+        EXPECT(set->add_device(ConstBytes{a.device, a.file.size()}, ImageSpec{a.type, a.w, a.h}, TileShape{64, 64},
+                               nullptr, nullptr, id) == Status::success);
+        if (i == 2) EXPECT(set->commit(nullptr) == Status::success);
+    }
+    // Rejected before any device work: host memory, a misaligned start.
+    {
+        std::uint32_t id = 0;
+        const Archive &a = archives[0];
+        EXPECT(set->add_device(ConstBytes{a.file.data(), a.file.size()}, ImageSpec{a.type, a.w, a.h},
+                               TileShape{64, 64}, nullptr, nullptr, id) == Status::invalid_argument);
+        EXPECT(set->add_device(ConstBytes{static_cast<const std::uint8_t *>(a.device) + 2, a.file.size() - 2},
+                               ImageSpec{a.type, a.w, a.h}, TileShape{64, 64}, nullptr, nullptr, id) ==
+               Status::invalid_argument);
+    }
     EXPECT(set->commit(nullptr) == Status::success);
     ServerOptions options;
     options.streams = 2;

@@ -227,6 +227,7 @@ PROTOTYPES = {
     "ts_archive_set_create": (_u32, [_P(_vp)]),
     "ts_archive_set_destroy": (None, [_vp]),
     "ts_archive_set_add": (_u32, [_vp, ConstBytes, _vp, _P(_u32)]),
+    "ts_archive_set_add_device": (_u32, [_vp, ConstBytes, ImageSpec, TileShape, _vp, _vp, _P(_u32)]),
     "ts_archive_set_commit": (_u32, [_vp, _vp]),
     "ts_archive_set_size": (_u32, [_vp]),
     "ts_archive_set_info": (_u32, [_vp, _u32, _P(ArchiveInfo)]),

@@ -598,6 +598,22 @@ class ArchiveSet:
             self._keep.append((device_file, metadata))
         return st, ident.value
 
+    def add_device(self, device_file, image: ImageSpec, tile, producer=None, size: Optional[int] = None,
+                   offset: int = 0, stream=None):
+        """Adds an archive in device memory (DeviceMemory, at `offset`) without
+        host metadata: validated and tabled on the device.  `producer`: the
+        encode's device result (a DeviceMemory or an address), or None when the
+        file is exactly `size` bytes.  Returns (status, id)."""
+        ident = C.c_uint32(0)
+        n = device_file.size - offset if size is None else size
+        prod = None if producer is None else (producer if isinstance(producer, int) else producer.ptr)
+        t = tile if isinstance(tile, TileShape) else TileShape(*tile)
+        st = Status(N.lib().ts_archive_set_add_device(self.handle, ConstBytes(device_file.ptr + offset, n), image, t,
+                                                      prod, _stream(stream), C.byref(ident)))
+        if st == Status.success:
+            self._keep.append((device_file, producer))
+        return st, ident.value
+
     def commit(self, stream=None) -> Status:
         return Status(N.lib().ts_archive_set_commit(self.handle, _stream(stream)))
 

@@ -31,7 +31,7 @@ HIP_FLAGS = ["-x", "hip", "--offload-arch=" + ARCH, "-fno-gpu-rdc",
 # Host-only sources: plain C++ against the HIP runtime API, no device pass.
 HOST_FLAGS = ["-x", "c++", "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include", "-Wall", "-Wextra"]
 
-HIP_SOURCES = ["hip/decode_block.hip", "hip/decode_mixed.hip", "hip/decode_u16.hip", "hip/decode_f32.hip", "hip/decode_u16_cal.hip",
+HIP_SOURCES = ["hip/decode_block.hip", "hip/index.hip", "hip/decode_mixed.hip", "hip/decode_u16.hip", "hip/decode_f32.hip", "hip/decode_u16_cal.hip",
                "hip/decode_f32_cal.hip", "hip/decode.hip", "hip/encode.hip", "hip/calibrate.hip"]
 HOST_SOURCES = ["host/sizing.cpp", "host/parser.cpp", "host/cpu_codec.cpp", "host/tiff.cpp", "host/plan.cpp",
                 "host/queue.cpp", "host/serve.cpp", "host/capi.cpp"]

@@ -238,6 +238,25 @@ __device__ inline void stage_tile(const tsk_encode_image &img, const Geometry &g
     }
 }
 
+// Window tile of workgroup b.  Workgroups are dispatched to the eight XCDs
+// round robin (b on XCD b % 8), and each XCD dispatches its own in increasing
+// order.  In every full group of 8 x run workgroups, XCD x takes the run of
+// tiles [x run, (x + 1) run) of the group, in increasing order; the tail
+// beyond the last full group is taken in order.  The rows of neighbouring
+// tiles share cache lines, which then stay in one L2.  Look-back safety: the
+// smallest unfinished tile T follows, in its XCD's order, only finished tiles
+// (earlier groups, and the earlier tiles of its own run), so its XCD has room
+// for it; workgroups that wait (tiles above T) and come before T in the
+// dispatch order belong to T's group and number fewer than run per XCD, so
+// with run at most 32 (one workgroup per CU of an XCD) they cannot fill an XCD
+// and stall the dispatch of T.
+__device__ inline std::uint32_t window_tile(std::uint32_t b, std::uint32_t n, std::uint32_t run) {
+    const std::uint32_t group = 8u * run;
+    if (run <= 1u || b >= n / group * group) return b;
+    const std::uint32_t base = b / group * group, r = b - base;
+    return base + (r % 8u) * run + r / 8u;
+}
+
 template <bool U16>
 __global__ void __launch_bounds__(Cfg<U16>::threads) __attribute__((amdgpu_waves_per_eu(Cfg<U16>::waves))) encode_kernel(tsk_encode_args a) {
     constexpr std::uint32_t kThreads = Cfg<U16>::threads;
@@ -252,7 +271,7 @@ __global__ void __launch_bounds__(Cfg<U16>::threads) __attribute__((amdgpu_waves
     __shared__ std::uint64_t shared_offset;
 
     const std::uint32_t lane = threadIdx.x % 64u;
-    const std::uint32_t t = blockIdx.x;
+    const std::uint32_t t = window_tile(blockIdx.x, a.tile_count, a.run);
     std::uint64_t *state = a.state + std::uint64_t{a.parity} * a.capacity;
     std::uint32_t o;
     if (a.blocks != nullptr) {

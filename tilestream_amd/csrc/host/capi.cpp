@@ -385,6 +385,12 @@ uint32_t ts_archive_set_add(ts_archive_set *set, ts_const_bytes file, const ts_m
     if (set == nullptr || metadata == nullptr || id == nullptr) return code(Status::invalid_argument);
     return code(set->set->add(as<ConstBytes>(file), *meta(metadata), *id));
 }
+uint32_t ts_archive_set_add_device(ts_archive_set *set, ts_const_bytes file, ts_image_spec image, ts_tile_shape tile,
+                                   const ts_device_result *producer, void *stream, uint32_t *id) {
+    if (set == nullptr || id == nullptr) return code(Status::invalid_argument);
+    return code(set->set->add_device(as<ConstBytes>(file), as<ImageSpec>(image), as<TileShape>(tile),
+                                     reinterpret_cast<const DeviceResult *>(producer), stream_of(stream), *id));
+}
 uint32_t ts_archive_set_commit(ts_archive_set *set, void *stream) {
     if (set == nullptr) return code(Status::invalid_argument);
     return code(set->set->commit(stream_of(stream)));

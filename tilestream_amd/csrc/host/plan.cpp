@@ -139,6 +139,13 @@ Status accessible(const void *data, int device, bool pageable) noexcept {
     }
 }
 
+// Memory of this device itself (not host, managed or another device's).
+Status on_device(const void *data, int device) noexcept {
+    Residence where = Residence::not_registered;
+    if (const Status st = residence_of(data, device, where); st != Status::success) return st;
+    return where == Residence::this_device ? Status::success : Status::invalid_argument;
+}
+
 // A call runs on the plan's device: it must be the calling thread's current
 // device and the stream's device.
 Status stream_device(hipStream_t stream, int device) noexcept {
@@ -852,6 +859,7 @@ namespace {
 Status enqueue_encode(const detail::PlanState &plan, std::uint64_t *state, std::uint64_t *carry, bool whole,
                       DeviceResult *results, hipStream_t stream) noexcept {
     tsk_encode_args a{};
+    a.run = detail::encode_run();
     a.images = plan.encode_images;
     a.results = results;
     a.state = state;

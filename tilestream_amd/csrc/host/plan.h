@@ -15,6 +15,12 @@ namespace tilestream::hip::detail {
 // Tiles per encode window: bounds the encoder's workspace for any batch size.
 inline constexpr std::uint32_t window_tiles = 4096;
 
+// Consecutive encode tiles per XCD (tsk_encode_args.run).  On the 256-image
+// batch, runs of 1, 2, 4, 8, 16, 32 tiles took 2.054, 2.041, 2.023, 2.077,
+// 2.284, 2.481 ms per step; runs of 8 cut the pixel bytes fetched from 1.172 x
+// to 1.053 x the pixels.
+inline constexpr std::uint32_t encode_run() noexcept { return 4u; }
+
 // A half-open address interval [lo, hi).
 struct Span {
     std::uintptr_t lo = 0;
@@ -135,6 +141,7 @@ inline Status hip_failure() noexcept {
 inline Status hip_status(int error) noexcept { return error == hipSuccess ? Status::success : hip_failure(); }
 
 [[nodiscard]] Status accessible(const void *data, int device, bool pageable) noexcept;
+[[nodiscard]] Status on_device(const void *data, int device) noexcept;
 [[nodiscard]] Status check_tile(const ImageMetadata &metadata, std::uint64_t index) noexcept;
 [[nodiscard]] Status stream_device(hipStream_t stream, int device) noexcept;
 

@@ -34,12 +34,22 @@ constexpr std::uint64_t align_up(std::uint64_t v, std::uint64_t a) { return (v +
 
 } // namespace
 
+// Tile table of an archive added from device memory, built there; copied
+// into the committed tables at every commit.
+struct DeviceTable {
+    void *storage = nullptr;           // tasks, then row offsets
+    std::uint64_t tile_base = 0, tiles = 0;
+    std::uint64_t rows_base = 0, rows = 0;
+    std::uint64_t rows_at = 0;         // byte offset of the row offsets in storage
+};
+
 struct ArchiveSet::Impl {
     int device = 0;
     bool pageable = false;
     std::vector<Archive> archives;
     std::vector<tsk_decode_task> tiles;
     std::vector<std::uint16_t> rows;
+    std::vector<DeviceTable> device_tables;
     // committed tables
     void *storage = nullptr;
     const tsk_decode_task *device_tiles = nullptr;
@@ -50,6 +60,7 @@ struct ArchiveSet::Impl {
 
     ~Impl() {
         if (storage != nullptr) static_cast<void>(hipFree(storage));
+        for (const DeviceTable &t : device_tables) static_cast<void>(hipFree(t.storage));
     }
 };
 
@@ -141,6 +152,103 @@ Status ArchiveSet::add(ConstBytes device_file, const ImageMetadata &m, std::uint
     }
 }
 
+Status ArchiveSet::add_device(ConstBytes device_file, const ImageSpec &image, TileShape tile,
+                              const DeviceResult *producer, hipStream_t stream, std::uint32_t &id) noexcept {
+    Impl &s = *impl_;
+    if (s.archives.size() >= UINT32_MAX) return Status::resource_limit;
+    std::uint64_t bound = 0;
+    if (const Status st = compress_bound(image, tile, bound); st != Status::success) return st;
+    if (const Status st = host::byte_range(device_file.data, device_file.size); st != Status::success) return st;
+    if (device_file.data == nullptr || reinterpret_cast<std::uintptr_t>(device_file.data) % 4 != 0)
+        return Status::invalid_argument;
+    if (const Status st = detail::stream_device(stream, s.device); st != Status::success) return st;
+    // Device memory only: tables of pinned host archives are built on the host.
+    if (const Status st = detail::on_device(device_file.data, s.device); st != Status::success) return st;
+    if (producer != nullptr &&
+        (reinterpret_cast<std::uintptr_t>(producer) % 8 != 0 ||
+         detail::accessible(producer, s.device, s.pageable) != Status::success))
+        return Status::invalid_argument;
+    const std::uint64_t across = host::ceil_div(image.width, tile.width);
+    const std::uint64_t count = across * host::ceil_div(image.height, tile.height);
+    const std::uint64_t rows = count * tile.height;
+    tsk_index_args a{};
+    a.src = static_cast<const std::uint8_t *>(device_file.data);
+    a.producer = producer;
+    a.capacity = device_file.size;
+    a.bound = bound;
+    a.tile_count = count;
+    a.width = image.width;
+    a.height = image.height;
+    a.tile_columns = static_cast<std::uint32_t>(across);
+    a.tile_w = tile.width;
+    a.tile_h = tile.height;
+    a.type = static_cast<std::uint32_t>(image.type);
+    // The header first: only a valid header (whose index lies inside the
+    // file) sizes the tables.
+    void *record_storage = nullptr;
+    if (hipMalloc(&record_storage, sizeof(tsk_index_record)) != hipSuccess) return detail::hip_failure();
+    a.record = static_cast<tsk_index_record *>(record_storage);
+    tsk_index_record record{};
+    const auto read_record = [&] {
+        return hipMemcpyAsync(&record, a.record, sizeof record, hipMemcpyDeviceToHost, stream) == hipSuccess &&
+               hipStreamSynchronize(stream) == hipSuccess;
+    };
+    DeviceTable table;
+    const auto release = [&](Status st) {
+        static_cast<void>(hipFree(record_storage));
+        if (table.storage != nullptr) static_cast<void>(hipFree(table.storage));
+        return st;
+    };
+    if (tsk_index_header(&a, stream) != hipSuccess || !read_record()) return release(detail::hip_failure());
+    if (record.header_status == TSK_INDEX_OTHER_IMAGE) return release(Status::invalid_argument);
+    if (record.header_status != 0) return release(static_cast<Status>(record.header_status));
+    if (s.tiles.size() + count > UINT32_MAX || s.rows.size() + rows > UINT32_MAX)
+        return release(Status::resource_limit);
+    table.tile_base = s.tiles.size();
+    table.tiles = count;
+    table.rows_base = s.rows.size();
+    table.rows = rows;
+    table.rows_at = align_up(count * sizeof(tsk_decode_task), 256);
+    if (hipMalloc(&table.storage, table.rows_at + rows * sizeof(std::uint16_t)) != hipSuccess)
+        return release(detail::hip_failure());
+    auto *base = static_cast<std::uint8_t *>(table.storage);
+    a.tasks = reinterpret_cast<tsk_decode_task *>(base);
+    a.rows = reinterpret_cast<std::uint16_t *>(base + table.rows_at);
+    a.rows_base = static_cast<std::uint32_t>(table.rows_base);
+    if (tsk_index_tiles(&a, stream) != hipSuccess || !read_record()) return release(detail::hip_failure());
+    if (record.tile_failures != 0) return release(Status::malformed_data);
+    static_cast<void>(hipFree(record_storage));
+    record_storage = nullptr;
+    try {
+        s.device_tables.push_back(table);
+        try {
+            s.tiles.resize(s.tiles.size() + count);
+            s.rows.resize(s.rows.size() + rows);
+            Archive ar;
+            ar.info.image = image;
+            ar.info.tile = tile;
+            ar.info.tile_count = count;
+            ar.info.file_bytes = record.file_bytes;
+            ar.info.host = 0;
+            ar.bytes = static_cast<const std::uint8_t *>(device_file.data);
+            ar.tile_base = table.tile_base;
+            ar.tile_columns = static_cast<std::uint32_t>(across);
+            s.archives.push_back(ar);
+        } catch (...) {
+            s.tiles.resize(table.tile_base);
+            s.rows.resize(table.rows_base);
+            s.device_tables.pop_back();
+            throw;
+        }
+    } catch (const std::bad_alloc &) {
+        return release(Status::resource_limit);
+    } catch (const std::length_error &) {
+        return release(Status::resource_limit);
+    }
+    id = static_cast<std::uint32_t>(s.archives.size() - 1);
+    return Status::success;
+}
+
 Status ArchiveSet::commit(hipStream_t stream) noexcept {
     Impl &s = *impl_;
     if (const Status st = detail::stream_device(stream, s.device); st != Status::success) return st;
@@ -149,11 +257,21 @@ Status ArchiveSet::commit(hipStream_t stream) noexcept {
     void *storage = nullptr;
     if (hipMalloc(&storage, total) != hipSuccess) return detail::hip_failure();
     auto *base = static_cast<std::uint8_t *>(storage);
-    if ((!s.tiles.empty() && hipMemcpyAsync(base, s.tiles.data(), s.tiles.size() * sizeof(tsk_decode_task),
-                                            hipMemcpyHostToDevice, stream) != hipSuccess) ||
-        (!s.rows.empty() && hipMemcpyAsync(base + rows_at, s.rows.data(), s.rows.size() * sizeof(std::uint16_t),
-                                           hipMemcpyHostToDevice, stream) != hipSuccess) ||
-        hipStreamSynchronize(stream) != hipSuccess) {
+    bool copied = (s.tiles.empty() || hipMemcpyAsync(base, s.tiles.data(), s.tiles.size() * sizeof(tsk_decode_task),
+                                                     hipMemcpyHostToDevice, stream) == hipSuccess) &&
+                  (s.rows.empty() || hipMemcpyAsync(base + rows_at, s.rows.data(), s.rows.size() * sizeof(std::uint16_t),
+                                                    hipMemcpyHostToDevice, stream) == hipSuccess);
+    // Tables built on the device replace their placeholders (same stream, after the upload).
+    for (const DeviceTable &t : s.device_tables) {
+        if (!copied) break;
+        const auto *from = static_cast<const std::uint8_t *>(t.storage);
+        copied = hipMemcpyAsync(base + t.tile_base * sizeof(tsk_decode_task), from, t.tiles * sizeof(tsk_decode_task),
+                                hipMemcpyDeviceToDevice, stream) == hipSuccess &&
+                 (t.rows == 0 || hipMemcpyAsync(base + rows_at + t.rows_base * sizeof(std::uint16_t), from + t.rows_at,
+                                                t.rows * sizeof(std::uint16_t), hipMemcpyDeviceToDevice,
+                                                stream) == hipSuccess);
+    }
+    if (!copied || hipStreamSynchronize(stream) != hipSuccess) {
         static_cast<void>(hipFree(storage));
         return detail::hip_failure();
     }
@@ -572,6 +690,7 @@ Status Server::Impl::launch(Slot &slot, const ServeRequest *requests, const std:
     }
     if (encodes[0] + encodes[1] != 0) {
         tsk_encode_args a{};
+        a.run = detail::encode_run();
         a.state = reinterpret_cast<std::uint64_t *>(slot.scratch + Scratch::state);
         a.carry = reinterpret_cast<std::uint64_t *>(slot.scratch + Scratch::carry);
         a.capacity = options.encode_window;
