@@ -249,6 +249,28 @@ def copy_ceiling(stream, total_bytes: int, steps: int = 5):
     return 2 * half / (ms * 1e-3) / 1e9
 
 
+def stream_ceilings(stream, read_bytes: int, write_bytes: int, steps: int = 5) -> dict:
+    """The one-sided HBM ceilings beside the copy: a read-only and a write-only
+    stream (tsk_stream_read / tsk_stream_write, each in its fastest shape
+    measured here) over 4 GiB, and the rate a kernel with this mix of reads
+    and writes would reach if its reads ran at the read-only rate and its
+    writes at the write-only rate one after the other (`mix_bound`)."""
+    from tilestream_amd import api
+    from tilestream_amd import _native as N
+    size = 4 << 30
+    a, sink = api.DeviceMemory(size), api.DeviceMemory(256)
+    a.fill(1)
+    rd = lambda: N.lib().tsk_stream_read(a.ptr, size // 16, sink.ptr, stream.handle)  # noqa: E731
+    wr = lambda: N.lib().tsk_stream_write(a.ptr, size // 16, 7, stream.handle)  # noqa: E731
+    rd()
+    wr()
+    read_gbps = size / (timed(stream, rd, steps) * 1e-3) / 1e9
+    write_gbps = size / (timed(stream, wr, steps) * 1e-3) / 1e9
+    seconds = read_bytes / (read_gbps * 1e9) + write_bytes / (write_gbps * 1e9)
+    return {"read_only": round(read_gbps, 1), "write_only": round(write_gbps, 1),
+            "mix_bound": round((read_bytes + write_bytes) / seconds / 1e9, 1)}
+
+
 WARM_MS = 0.0  # --warm-ms: settle() before each measured section
 
 
@@ -1520,6 +1542,8 @@ def main() -> int:
 
     ceiling = copy_ceiling(stream, stats.payload_bytes + stats.output_bytes + stats.map_bytes)
     log(f"copy ceiling for the same bytes: {ceiling:.1f} GB/s")
+    one_sided = stream_ceilings(stream, stats.payload_bytes + stats.map_bytes, stats.output_bytes)
+    log(f"one-sided ceilings: {one_sided}")
     cpu = None if args.no_cpu else cpu_baseline(batch, threads, 1_000_000_000)
     del plans, pws, rec
     roi = None if args.no_roi else roi_leg(args, threads, stream)
@@ -1573,12 +1597,15 @@ def main() -> int:
             "algorithmic_bytes_per_launch": traffic_alg,
             "copy_ceiling": round(ceiling, 1),
             "frac_of_copy_ceiling": round(achieved / ceiling, 4),
+            "stream_ceilings": one_sided,
+            "frac_of_mix_bound": round(achieved / one_sided["mix_bound"], 4),
             "note": "payload bytes of every tile read + decoded output bytes written, over the "
                     "graph launch time (results init, counter reset, decode, results completion; the "
                     "decode kernel is 98-99% of it); traffic = HBM bytes per launch from rocprofv3 "
                     "FETCH_SIZE/WRITE_SIZE on this workload (profiles/decode_batch256_kernel_trace.md); "
                     "copy_ceiling = a plain 16-byte streaming copy moving the same number of bytes, "
-                    "measured in this run",
+                    "measured in this run; stream_ceilings = read-only and write-only streams of 4 GiB "
+                    "and the rate of this launch's reads and writes at those two rates in sequence",
         },
         "autotune": {"group": best.group, "block": best.block, "chunk": best.chunk, "depth": best.depth,
                      "candidates": len(candidates)},

@@ -294,6 +294,11 @@ int tsk_copy_blocks(void *dst, const void *src, uint64_t blocks, void *stream);
 /* Copies `blocks` 16-byte blocks in the fastest plain streaming shape measured
  * on this hardware: the practical ceiling for kernels that read and write. */
 int tsk_stream_copy(void *dst, const void *src, uint64_t blocks, void *stream);
+/* Reads `blocks` 16-byte blocks (sink: one word, written only if the XOR of
+ * all data is all-ones) and writes `blocks` 16-byte blocks of `value`, each in
+ * its fastest streaming shape measured here: the one-sided HBM ceilings. */
+int tsk_stream_read(const void *src, uint64_t blocks, void *sink, void *stream);
+int tsk_stream_write(void *dst, uint64_t blocks, uint32_t value, void *stream);
 /* Zero-fills `words` 32-bit words (a kernel, safe under graph capture). */
 int tsk_fill_u32(uint32_t *data, uint64_t words, uint32_t value, void *stream);
 

@@ -27,7 +27,9 @@ def test_host_code_under_sanitizer(tmp_path, sanitizer):
                             "-I" + os.path.join(ROOT, "include"), "-I" + HOST] + SOURCES + ["-o", exe, "-lz"],
                            capture_output=True, text=True, timeout=300)
     assert build.returncode == 0, build.stderr[-3000:]
-    env = dict(os.environ, TSAN_OPTIONS="halt_on_error=1", ASAN_OPTIONS="detect_leaks=1:halt_on_error=1")
+    # Each run writes its TIFF files under its own directory: parallel runs must not share them.
+    env = dict(os.environ, TSAN_OPTIONS="halt_on_error=1", ASAN_OPTIONS="detect_leaks=1:halt_on_error=1",
+               TMPDIR=str(tmp_path))
     run = subprocess.run([exe], capture_output=True, text=True, timeout=600, env=env)
     assert run.returncode == 0, (run.stdout[-2000:], run.stderr[-3000:])
     assert "Sanitizer" not in run.stderr, run.stderr[-3000:]

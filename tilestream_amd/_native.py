@@ -261,6 +261,8 @@ PROTOTYPES = {
     "ts_abi_sizeof": (_u64, [C.c_char_p]),
     "tsk_copy_blocks": (_int, [_vp, _vp, _u64, _vp]),
     "tsk_stream_copy": (_int, [_vp, _vp, _u64, _vp]),
+    "tsk_stream_read": (_int, [_vp, _u64, _vp, _vp]),
+    "tsk_stream_write": (_int, [_vp, _u64, _u32, _vp]),
     "tsk_fill_u32": (_int, [_vp, _u64, _u32, _vp]),
     "tsk_decode_supported": (_int, [_u32, _u32, _u32]),
     "tsk_decode_occupancy": (_int, [_u32, _u32, _u32, _u32, _P(_int)]),

@@ -288,6 +288,9 @@ __device__ inline void block_rows_full(const tsk_decode_image &img, const BlockJ
     const std::uint32_t col0 = p * G;
     const std::uint64_t ex0 = std::uint64_t{j.col0 + kc} * W - img.roi_x + col0;
     const std::uint64_t ey0 = std::uint64_t{j.row0} * H - img.roi_y;
+    // The output row of this wave, advanced by `waves` rows per step.
+    gptr<std::uint8_t> line = global(img.out) + (ey0 + wave) * img.out_pitch;
+    const std::uint64_t line_step = std::uint64_t{WAVES} * img.out_pitch;
 #pragma unroll
     for (int i = 0; i < PER_WAVE; ++i) {
         const std::uint32_t y = wave + static_cast<std::uint32_t>(i) * WAVES;
@@ -298,8 +301,8 @@ __device__ inline void block_rows_full(const tsk_decode_image &img, const BlockJ
         Group<G> v;
         std::uint32_t incl;
         predict_group<G, U16>(rb, col0, p, S, rb.seed, v, incl);
-        const std::uint64_t ey = ey0 + y;
-        store_group<G, U16, CAL, NT>(img, global(img.out) + ey * img.out_pitch, ex0, ey, v);
+        store_group<G, U16, CAL, NT>(img, line, ex0, ey0 + y, v);
+        line += line_step;
     }
 }
 

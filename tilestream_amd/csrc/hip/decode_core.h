@@ -323,19 +323,44 @@ __device__ inline void predict_group(const RowBits &rb, std::uint32_t col0, std:
     const std::uint32_t bit0 = __umul24(col0, rb.b) + rb.words_bit;
     std::uint32_t acc = 0;
     Group<G> pre;
+    if constexpr (U16 && G % 2 == 0) {
+        // Two uint16 residuals take 2b <= 32 bits, which always lie in the
+        // 64-bit window of the word holding the first one and the next: one
+        // LDS read and one funnel shift serve the pair, the second residual
+        // starting at bit b of it.
+        std::uint32_t bit = bit0;
+        const std::uint32_t step = rb.b * 2u;
 #pragma unroll
-    for (int j = 0; j < G; ++j) {
-        const std::uint32_t bit = __umul24(static_cast<std::uint32_t>(j), rb.b) + bit0;
-        lds_word *w = lds_at((bit >> 3) & ~3u);
-        const std::uint32_t x = __builtin_amdgcn_alignbit(w[1], w[0], bit);
-        if constexpr (U16) {
-            const std::uint32_t half = __builtin_amdgcn_ubfe(x, 1u, rb.high);
-            const std::uint32_t sign = static_cast<std::uint32_t>(__builtin_amdgcn_sbfe(static_cast<int>(x), 0u, rb.low));
-            acc += half ^ sign;
-        } else {
-            acc ^= x & rb.mask;
+        for (int j = 0; j < G; j += 2) {
+            lds_word *w = lds_at((bit >> 3) & ~3u);
+            const std::uint32_t x = __builtin_amdgcn_alignbit(w[1], w[0], bit);
+            const std::uint32_t half0 = __builtin_amdgcn_ubfe(x, 1u, rb.high);
+            const std::uint32_t sign0 = static_cast<std::uint32_t>(__builtin_amdgcn_sbfe(static_cast<int>(x), 0u, rb.low));
+            acc += half0 ^ sign0;
+            pre.v[j] = acc;
+            const std::uint32_t half1 = __builtin_amdgcn_ubfe(x, rb.b + 1u, rb.high);
+            const std::uint32_t sign1 =
+                static_cast<std::uint32_t>(__builtin_amdgcn_sbfe(static_cast<int>(x), rb.b, rb.low));
+            acc += half1 ^ sign1;
+            pre.v[j + 1] = acc;
+            bit += step;
         }
-        pre.v[j] = acc;
+    } else {
+#pragma unroll
+        for (int j = 0; j < G; ++j) {
+            const std::uint32_t bit = __umul24(static_cast<std::uint32_t>(j), rb.b) + bit0;
+            lds_word *w = lds_at((bit >> 3) & ~3u);
+            const std::uint32_t x = __builtin_amdgcn_alignbit(w[1], w[0], bit);
+            if constexpr (U16) {
+                const std::uint32_t half = __builtin_amdgcn_ubfe(x, 1u, rb.high);
+                const std::uint32_t sign =
+                    static_cast<std::uint32_t>(__builtin_amdgcn_sbfe(static_cast<int>(x), 0u, rb.low));
+                acc += half ^ sign;
+            } else {
+                acc ^= x & rb.mask;
+            }
+            pre.v[j] = acc;
+        }
     }
     incl = segment_scan<U16>(acc, p, S);
     const std::uint32_t first = Ring<U16>::add(carry, Ring<U16>::sub(incl, acc));
