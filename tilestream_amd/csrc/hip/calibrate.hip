@@ -1,8 +1,11 @@
 // Standalone calibration y = RN(RN(x - offset) * gain) for gfx950
 // (reference src/cuda/calibrate.cu:13-62; docs/numerical-analysis.md).
 // Rows of all images are concatenated; a workgroup takes rows with a stride,
-// finds the row's image by binary search over first_rows, and its lanes stream
-// the row one pixel per lane per step (consecutive lanes, consecutive pixels).
+// finds the row's image by binary search over first_rows, and its lanes
+// stream the row four pixels at a time: 16-byte accesses to the float32
+// input, the maps and the output (word alignment suffices for multi-word
+// accesses), 8 bytes of uint16 input when the input row starts on a word
+// (four 2-byte loads otherwise); the last width % 4 pixels one per lane.
 #include <hip/hip_runtime.h>
 
 #include <tilestream_kernels.h>
@@ -17,12 +20,28 @@ struct DeviceResultBits {
     std::uint64_t bytes;
 };
 
-template <bool U16> __device__ inline float sample(const std::uint8_t *row, std::uint32_t x) {
-    if constexpr (U16) return static_cast<float>(reinterpret_cast<const std::uint16_t *>(row)[x]);
-    else return reinterpret_cast<const float *>(row)[x];
+template <typename T> using gptr = T __attribute__((address_space(1))) *;
+// Vectors of words that need only word alignment.
+typedef float f32x4w __attribute__((ext_vector_type(4), aligned(4)));
+typedef std::uint32_t u32x2w __attribute__((ext_vector_type(2), aligned(4)));
+
+template <bool U16> __device__ inline float sample(gptr<const std::uint8_t> row, std::uint32_t x) {
+    if constexpr (U16) return static_cast<float>(reinterpret_cast<gptr<const std::uint16_t>>(row)[x]);
+    else return reinterpret_cast<gptr<const float>>(row)[x];
 }
 
+// Two roundings to nearest even, never contracted into an FMA.
+__device__ inline float calibrate(float x, float o, float g) { return __fmul_rn(__fsub_rn(x, o), g); }
+
 template <bool U16> __global__ void __launch_bounds__(256) calibrate_kernel(tsk_calibrate_args a) {
+    // The result records: the host checked every image before the launch, so
+    // each succeeds with its FP32 output bytes; they are visible to the caller
+    // when the kernel has completed, like the outputs.
+    if (blockIdx.x == 0) {
+        auto *results = static_cast<DeviceResultBits *>(a.results);
+        for (std::uint32_t i = threadIdx.x; i < a.image_count; i += blockDim.x)
+            results[i] = {0u, 0u, std::uint64_t{a.images[i].width} * a.images[i].height * 4u};
+    }
     for (std::uint64_t row = blockIdx.x; row < a.total_rows; row += gridDim.x) {
         std::uint32_t lo = 0, hi = a.image_count;
         while (hi - lo > 1) {
@@ -32,22 +51,46 @@ template <bool U16> __global__ void __launch_bounds__(256) calibrate_kernel(tsk_
         }
         const tsk_calibrate_image &img = a.images[lo];
         const std::uint64_t y = row - a.first_rows[lo];
-        const std::uint8_t *in = img.in + y * img.in_pitch;
-        float *out = reinterpret_cast<float *>(img.out + y * img.out_pitch);
-        const float *om = img.offset_map ? reinterpret_cast<const float *>(img.offset_map + y * img.offset_pitch) : nullptr;
-        const float *gm = img.gain_map ? reinterpret_cast<const float *>(img.gain_map + y * img.gain_pitch) : nullptr;
-        for (std::uint32_t x = threadIdx.x; x < img.width; x += blockDim.x) {
-            const float o = om ? om[x] : img.offset;
-            const float g = gm ? gm[x] : img.gain;
-            out[x] = __fmul_rn(__fsub_rn(sample<U16>(in, x), o), g);
+        const auto in = reinterpret_cast<gptr<const std::uint8_t>>(reinterpret_cast<std::uintptr_t>(img.in + y * img.in_pitch));
+        const auto out = reinterpret_cast<gptr<float>>(reinterpret_cast<std::uintptr_t>(img.out + y * img.out_pitch));
+        const auto om = img.offset_map ? reinterpret_cast<gptr<const float>>(reinterpret_cast<std::uintptr_t>(
+                                              img.offset_map + y * img.offset_pitch))
+                                        : nullptr;
+        const auto gm = img.gain_map ? reinterpret_cast<gptr<const float>>(reinterpret_cast<std::uintptr_t>(
+                                            img.gain_map + y * img.gain_pitch))
+                                      : nullptr;
+        const std::uint32_t width = img.width, groups = width / 4;
+        const bool in_words = !U16 || reinterpret_cast<std::uintptr_t>(in) % 4 == 0;
+#pragma unroll 2
+        for (std::uint32_t q = threadIdx.x; q < groups; q += blockDim.x) {
+            const std::uint32_t x = 4 * q;
+            float v[4];
+            if constexpr (U16) {
+                if (in_words) {
+                    const u32x2w w = __builtin_nontemporal_load(reinterpret_cast<gptr<const u32x2w>>(in + 2 * x));
+                    v[0] = static_cast<float>(w.x & 0xffffu);
+                    v[1] = static_cast<float>(w.x >> 16);
+                    v[2] = static_cast<float>(w.y & 0xffffu);
+                    v[3] = static_cast<float>(w.y >> 16);
+                } else {
+#pragma unroll
+                    for (int k = 0; k < 4; ++k) v[k] = sample<true>(in, x + k);
+                }
+            } else {
+                const f32x4w w = __builtin_nontemporal_load(reinterpret_cast<gptr<const f32x4w>>(in + 4 * x));
+                v[0] = w.x, v[1] = w.y, v[2] = w.z, v[3] = w.w;
+            }
+            f32x4w o = {img.offset, img.offset, img.offset, img.offset};
+            f32x4w g = {img.gain, img.gain, img.gain, img.gain};
+            if (om) o = __builtin_nontemporal_load(reinterpret_cast<gptr<const f32x4w>>(om + x));
+            if (gm) g = __builtin_nontemporal_load(reinterpret_cast<gptr<const f32x4w>>(gm + x));
+            const f32x4w r = {calibrate(v[0], o.x, g.x), calibrate(v[1], o.y, g.y), calibrate(v[2], o.z, g.z),
+                              calibrate(v[3], o.w, g.w)};
+            __builtin_nontemporal_store(r, reinterpret_cast<gptr<f32x4w>>(out + x));
         }
+        const std::uint32_t x = 4 * groups + threadIdx.x;
+        if (x < width) out[x] = calibrate(sample<U16>(in, x), om ? om[x] : img.offset, gm ? gm[x] : img.gain);
     }
-}
-
-__global__ void report_kernel(const tsk_calibrate_image *images, std::uint32_t count, DeviceResultBits *results) {
-    const std::uint64_t stride = std::uint64_t{gridDim.x} * blockDim.x;
-    for (std::uint64_t i = std::uint64_t{blockIdx.x} * blockDim.x + threadIdx.x; i < count; i += stride)
-        results[i] = {0u, 0u, std::uint64_t{images[i].width} * images[i].height * 4u};
 }
 
 } // namespace
@@ -60,9 +103,6 @@ int tsk_calibrate(const tsk_calibrate_args *a, void *stream) {
     const std::uint32_t grid = static_cast<std::uint32_t>(a->total_rows < 65536 ? a->total_rows : 65536);
     if (a->type == TSK_UINT16) hipLaunchKernelGGL(calibrate_kernel<true>, dim3(grid), dim3(256), 0, s, *a);
     else hipLaunchKernelGGL(calibrate_kernel<false>, dim3(grid), dim3(256), 0, s, *a);
-    const std::uint32_t reports = a->image_count / 256 + 1 < 1024 ? a->image_count / 256 + 1 : 1024;
-    hipLaunchKernelGGL(report_kernel, dim3(reports), dim3(256), 0, s, a->images, a->image_count,
-                       static_cast<DeviceResultBits *>(a->results));
     return hipGetLastError();
 }
 
@@ -70,7 +110,6 @@ int tsk_preload_calibrate(void) {
     hipFuncAttributes attr;
     hipError_t e = hipFuncGetAttributes(&attr, reinterpret_cast<const void *>(calibrate_kernel<true>));
     if (e == hipSuccess) e = hipFuncGetAttributes(&attr, reinterpret_cast<const void *>(calibrate_kernel<false>));
-    if (e == hipSuccess) e = hipFuncGetAttributes(&attr, reinterpret_cast<const void *>(report_kernel));
     return e;
 }
 
