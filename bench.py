@@ -1429,8 +1429,9 @@ def main() -> int:
     st, queue = api.DecodeQueue.prepare(items, [api.Operation.decompress] * n, stream)
     assert st == api.Status.success, st
     qws = api.DeviceMemory(max(queue.workspace_bytes(), 256))
-    candidates = [(g, b, 0, c, d) for g in (4, 8, 16) for (b, d) in ((128, 1), (128, 2), (256, 2), (256, 3),
-                                                                    (512, 2), (512, 3))
+    # The tile decoder's whole space: lane-group width x workgroup size x depth,
+    # each at queue chunks of 4 and 16 tiles, plus two static-stride shapes.
+    candidates = [(g, b, 0, c, d) for g in (1, 2, 4, 8, 16) for b in (64, 128, 256, 512) for d in (1, 2, 3)
                   for c in (4, 16)]
     candidates += [(8, 256, 0, 0, 2), (4, 256, 0, 0, 3)]
     # The block decoder (group 0): 512 threads, two workgroups per CU.

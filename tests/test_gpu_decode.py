@@ -37,8 +37,7 @@ SHAPE_GRID = [(g_, b_, d_) for g_ in (1, 2, 4, 8, 16) for b_ in (64, 128, 256, 5
 
 @pytest.mark.parametrize("group,block,depth", SHAPE_GRID)
 def test_lane_group_widths(orc, group, block, depth):
-    if not ts.api.decode_shape_supported(group, block, depth):
-        pytest.skip("no kernel compiled for this shape")
+    assert ts.api.decode_shape_supported(group, block, depth)
     for kind, tile in [("u16", (128, 32)), ("f32", (64, 64)), ("u16", (256, 16))]:
         px = g.detector_u16(300, 200, 9) if kind == "u16" else g.field_f32(300, 200, 9)
         data = orc.compress(px, tile)

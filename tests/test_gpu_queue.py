@@ -62,7 +62,8 @@ def test_whole_batch_matches_pixels(batch):
     n = len(batch.entries)
     for shape in [(8, 256, 0, 1, 2), (4, 128, 0, 4, 1), (16, 512, 0, 2, 2), (8, 256, 0, 0, 2),
                   (2, 256, 37, 1, 1), (4, 512, 5, 3, 2), (8, 64, 0, 16, 1), (4, 256, 0, 4, 3),
-                  (8, 512, 7, 2, 3), (16, 256, 0, 0, 3), (4, 128, 0, 4, 2)] + BLOCK_SHAPES:
+                  (8, 512, 7, 2, 3), (16, 256, 0, 0, 3), (4, 128, 0, 4, 2), (1, 64, 0, 4, 3),
+                  (2, 128, 0, 16, 2), (1, 512, 3, 1, 2), (2, 64, 0, 0, 3)] + BLOCK_SHAPES:
         batch.clear_outputs()
         ops = [ts.Operation.decompress] * n
         rois = [(0, 0, e.width, e.height) for e in batch.entries]
@@ -97,7 +98,7 @@ def test_graph_replays_are_identical(batch):
             assert np.array_equal(batch.read_output(i), e.pixels)
 
 
-@pytest.mark.parametrize("shape", [None, (0, 512, 0, 2, 2)])
+@pytest.mark.parametrize("shape", [None, (0, 512, 0, 2, 2), (2, 256, 0, 4, 3)])
 def test_mixed_regions_and_calibration_alone_and_in_mix(batch, orc, shape):
     rng = np.random.default_rng(78)
     n = len(batch.entries)

@@ -31,16 +31,23 @@ HIP_FLAGS = ["-x", "hip", "--offload-arch=" + ARCH, "-fno-gpu-rdc",
 # Host-only sources: plain C++ against the HIP runtime API, no device pass.
 HOST_FLAGS = ["-x", "c++", "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include", "-Wall", "-Wextra"]
 
-HIP_SOURCES = ["hip/decode_block.hip", "hip/index.hip", "hip/decode_mixed.hip", "hip/decode_u16.hip", "hip/decode_f32.hip", "hip/decode_u16_cal.hip",
-               "hip/decode_f32_cal.hip", "hip/decode.hip", "hip/encode.hip", "hip/calibrate.hip"]
+# The tile decoder's kernels: one source per kind, compiled once per lane-group
+# width (the object name carries the width).
+DECODE_KINDS = ["mixed", "u16", "f32", "u16_cal", "f32_cal"]
+GROUPS = [1, 2, 4, 8, 16]
+HIP_SOURCES = ([f"hip/decode_{k}.hip:{g}" for k in DECODE_KINDS for g in GROUPS] +
+               ["hip/decode_block.hip", "hip/index.hip", "hip/decode.hip", "hip/encode.hip", "hip/calibrate.hip"])
 HOST_SOURCES = ["host/sizing.cpp", "host/parser.cpp", "host/cpu_codec.cpp", "host/tiff.cpp", "host/plan.cpp",
                 "host/queue.cpp", "host/serve.cpp", "host/capi.cpp"]
 
 
-def _compile(src: str) -> str:
+def _compile(entry: str) -> str:
+    src, _, group = entry.partition(":")
     path = os.path.join(CSRC, src)
-    obj = os.path.join(OBJ, src.replace("/", "_") + ".o")
+    obj = os.path.join(OBJ, src.replace("/", "_") + (f"_g{group}" if group else "") + ".o")
     flags = COMMON + (HIP_FLAGS if src.endswith(".hip") else HOST_FLAGS)
+    if group:
+        flags = flags + [f"-DTSK_DECODE_GROUP={group}"]
     if src.startswith("hip/decode_"):
         # The work queue's look-ahead draw is one lane's atomic whose result is
         # awaited later: keep it a plain returning atomic, not a wave-combined

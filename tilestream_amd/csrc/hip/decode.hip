@@ -6,12 +6,22 @@
 using tsk::DecodeFn;
 using tsk::ResultBits;
 
+// The kernel lookups of each kind, one per lane-group width 1, 2, 4, 8, 16.
+#define TSK_DECLARE_TABLE(KIND_NAME, G) DecodeFn TSK_TABLE_NAME(KIND_NAME, G)(std::uint32_t block, std::uint32_t depth);
+#define TSK_DECLARE_TABLES(KIND_NAME)                                                                  \
+    TSK_DECLARE_TABLE(KIND_NAME, 1)                                                                   \
+    TSK_DECLARE_TABLE(KIND_NAME, 2)                                                                   \
+    TSK_DECLARE_TABLE(KIND_NAME, 4) TSK_DECLARE_TABLE(KIND_NAME, 8) TSK_DECLARE_TABLE(KIND_NAME, 16)
+#define TSK_TABLE_ROW(KIND_NAME)                                                                       \
+    {TSK_TABLE_NAME(KIND_NAME, 1), TSK_TABLE_NAME(KIND_NAME, 2), TSK_TABLE_NAME(KIND_NAME, 4),            \
+     TSK_TABLE_NAME(KIND_NAME, 8), TSK_TABLE_NAME(KIND_NAME, 16)}
+
 extern "C" {
-DecodeFn tsk_decode_table_u16(std::uint32_t group, std::uint32_t block, std::uint32_t depth);
-DecodeFn tsk_decode_table_f32(std::uint32_t group, std::uint32_t block, std::uint32_t depth);
-DecodeFn tsk_decode_table_u16_cal(std::uint32_t group, std::uint32_t block, std::uint32_t depth);
-DecodeFn tsk_decode_table_f32_cal(std::uint32_t group, std::uint32_t block, std::uint32_t depth);
-DecodeFn tsk_decode_table_mixed(std::uint32_t group, std::uint32_t block, std::uint32_t depth);
+TSK_DECLARE_TABLES(u16)
+TSK_DECLARE_TABLES(f32)
+TSK_DECLARE_TABLES(u16_cal)
+TSK_DECLARE_TABLES(f32_cal)
+TSK_DECLARE_TABLES(mixed)
 DecodeFn tsk_block_table(std::uint32_t kind, std::uint32_t block);
 }
 
@@ -138,15 +148,15 @@ __global__ void fill_kernel(std::uint32_t *data, std::uint64_t words, std::uint3
 
 DecodeFn decode_function(std::uint32_t kind, std::uint32_t group, std::uint32_t block, std::uint32_t depth) {
     if (group == 0) return depth == 2 ? tsk_block_table(kind, block) : nullptr;
-    if (!tsk::compiled(group, block, depth)) return nullptr;
-    switch (kind) {
-    case TSK_KIND_U16: return tsk_decode_table_u16(group, block, depth);
-    case TSK_KIND_F32: return tsk_decode_table_f32(group, block, depth);
-    case TSK_KIND_U16_CAL: return tsk_decode_table_u16_cal(group, block, depth);
-    case TSK_KIND_F32_CAL: return tsk_decode_table_f32_cal(group, block, depth);
-    case TSK_KIND_MIXED: return tsk_decode_table_mixed(group, block, depth);
-    default: return nullptr;
-    }
+    using Table = DecodeFn (*)(std::uint32_t, std::uint32_t);
+    // Rows in TSK_KIND_* order.
+    static constexpr Table tables[5][5] = {TSK_TABLE_ROW(u16), TSK_TABLE_ROW(f32), TSK_TABLE_ROW(u16_cal),
+                                           TSK_TABLE_ROW(f32_cal), TSK_TABLE_ROW(mixed)};
+    static_assert(TSK_KIND_U16 == 0 && TSK_KIND_F32 == 1 && TSK_KIND_U16_CAL == 2 && TSK_KIND_F32_CAL == 3 &&
+                  TSK_KIND_MIXED == 4);
+    if (!tsk::compiled(group, block, depth) || kind > TSK_KIND_MIXED) return nullptr;
+    const int width = group == 1 ? 0 : group == 2 ? 1 : group == 4 ? 2 : group == 8 ? 3 : 4;
+    return tables[kind][width](block, depth);
 }
 
 std::uint32_t small_grid(std::uint64_t n) {

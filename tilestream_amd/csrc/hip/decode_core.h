@@ -442,9 +442,10 @@ __device__ void decode_tile(const tsk_decode_image &img, const TileView &t, cons
     }
 }
 
-// Groups per thread of a 4096-pixel tile when G >= 4 (one pass per row).
+// Groups per thread of a 4096-pixel tile when G >= 4 (one pass per row); the
+// narrower groups store as they decode and hold nothing.
 template <int G, int BLOCK> constexpr int held_groups() {
-    return (4096 + BLOCK * G - 1) / (BLOCK * G);
+    return G >= 4 ? (4096 + BLOCK * G - 1) / (BLOCK * G) : 1;
 }
 
 // A thread's decoded groups of one tile, kept in registers until they are
@@ -745,6 +746,27 @@ __device__ inline void prefetch(const tsk_decode_args &a, const Pending &p, u32x
     row_reg = rows ? v : 0u;
 }
 
+// Decodes the staged tile with the kernel's kind and stores it as it goes.
+template <int G, int KIND, int BLOCK>
+__device__ inline void decode_tile_as(const tsk_decode_image &img, const TileView &t, const std::uint32_t *stage,
+                                      const std::uint16_t *row_start) {
+    if constexpr (KIND == TSK_KIND_U16) {
+        decode_tile<G, true, false, BLOCK>(img, t, stage, row_start);
+    } else if constexpr (KIND == TSK_KIND_F32) {
+        decode_tile<G, false, false, BLOCK>(img, t, stage, row_start);
+    } else if constexpr (KIND == TSK_KIND_U16_CAL) {
+        decode_tile<G, true, true, BLOCK>(img, t, stage, row_start);
+    } else if constexpr (KIND == TSK_KIND_F32_CAL) {
+        decode_tile<G, false, true, BLOCK>(img, t, stage, row_start);
+    } else if (img.type == TSK_UINT16) {
+        if (img.calibrated) decode_tile<G, true, true, BLOCK>(img, t, stage, row_start);
+        else decode_tile<G, true, false, BLOCK>(img, t, stage, row_start);
+    } else {
+        if (img.calibrated) decode_tile<G, false, true, BLOCK>(img, t, stage, row_start);
+        else decode_tile<G, false, false, BLOCK>(img, t, stage, row_start);
+    }
+}
+
 // Depth 1: the payload of a tile is loaded after the previous one is decoded.
 template <int G, int KIND, int BLOCK>
 __device__ inline __attribute__((always_inline)) void decode_body_staged(const tsk_decode_args &a) {
@@ -777,22 +799,7 @@ __device__ inline __attribute__((always_inline)) void decode_body_staged(const t
             if (tid == 0) fail(a.results, c.image, kMalformed);
             continue;
         }
-        const TileView t = tile_view(c, img);
-        if constexpr (KIND == TSK_KIND_U16) {
-            decode_tile<G, true, false, BLOCK>(img, t, stage, row_start);
-        } else if constexpr (KIND == TSK_KIND_F32) {
-            decode_tile<G, false, false, BLOCK>(img, t, stage, row_start);
-        } else if constexpr (KIND == TSK_KIND_U16_CAL) {
-            decode_tile<G, true, true, BLOCK>(img, t, stage, row_start);
-        } else if constexpr (KIND == TSK_KIND_F32_CAL) {
-            decode_tile<G, false, true, BLOCK>(img, t, stage, row_start);
-        } else if (u16) {
-            if (img.calibrated) decode_tile<G, true, true, BLOCK>(img, t, stage, row_start);
-            else decode_tile<G, true, false, BLOCK>(img, t, stage, row_start);
-        } else {
-            if (img.calibrated) decode_tile<G, false, true, BLOCK>(img, t, stage, row_start);
-            else decode_tile<G, false, false, BLOCK>(img, t, stage, row_start);
-        }
+        decode_tile_as<G, KIND, BLOCK>(img, tile_view(c, img), stage, row_start);
     }
 }
 
@@ -815,7 +822,11 @@ __device__ inline __attribute__((always_inline)) bool decode_phase(const tsk_dec
         return false;
     }
     t = tile_view(c, img);
-    if constexpr (KIND == TSK_KIND_U16 || KIND == TSK_KIND_U16_CAL) {
+    if constexpr (G < 4) {
+        // A row may take several passes: decoded and stored as it goes.
+        decode_tile_as<G, KIND, BLOCK>(img, t, stage, row_start);
+        return false;
+    } else if constexpr (KIND == TSK_KIND_U16 || KIND == TSK_KIND_U16_CAL) {
         decode_tile_held<G, true, BLOCK, K>(img, t, stage, row_start, h);
     } else if constexpr (KIND == TSK_KIND_F32 || KIND == TSK_KIND_F32_CAL) {
         decode_tile_held<G, false, BLOCK, K>(img, t, stage, row_start, h);
@@ -903,7 +914,9 @@ struct Pipeline {
             await_prefetch(reg[S], row_reg[S], cursor.ahead);
             commit<BLOCK>(a, p[0], reg[S], row_reg[S], stage[buf ^ 1u], row_start[buf ^ 1u]);
         }
-        if (store) store_phase<G, KIND, BLOCK, K>(img, t, h);
+        if constexpr (G >= 4) {
+            if (store) store_phase<G, KIND, BLOCK, K>(img, t, h);
+        }
         if (!had_next) return false;
         c = p[0];
         if constexpr (SETS == 2) {
@@ -967,66 +980,43 @@ __global__ void __launch_bounds__(BLOCK) decode_kernel(tsk_decode_args a) {
 
 using DecodeFn = void (*)(tsk_decode_args);
 
-// The (group, block, depth) combinations that are compiled: every group with
-// 256 threads and depth 1; groups 4, 8, 16 with every block size at depth 1,
-// 128, 256 and 512 threads at depth 2 and 256 and 512 threads at depth 3
-// (each register set holds ceil(1025 / block) 16-byte blocks per thread).
+// Every (group, block, depth) combination is compiled: groups 1, 2, 4, 8, 16
+// lanes per row, 64 to 512 threads, depth 1 to 3 (each register set of depth
+// 2 and 3 holds ceil(1025 / block) 16-byte blocks per thread).
 constexpr bool compiled(std::uint32_t group, std::uint32_t block, std::uint32_t depth) {
     const bool g_ok = group == 1 || group == 2 || group == 4 || group == 8 || group == 16;
     const bool b_ok = block == 64 || block == 128 || block == 256 || block == 512;
-    if (!g_ok || !b_ok || depth < 1 || depth > 3) return false;
-    if (group <= 2) return block == 256 && depth == 1;
-    if (depth == 3) return block >= 256;
-    return depth == 1 || block >= 128;
+    return g_ok && b_ok && depth >= 1 && depth <= 3;
 }
 
-template <int KIND, int G> DecodeFn pick_block(std::uint32_t block, std::uint32_t depth) {
-    if constexpr (G <= 2) {
-        return block == 256 && depth == 1 ? decode_kernel<G, KIND, 256, 1> : nullptr;
-    } else {
-        if (depth == 1) {
-            switch (block) {
-            case 64: return decode_kernel<G, KIND, 64, 1>;
-            case 128: return decode_kernel<G, KIND, 128, 1>;
-            case 256: return decode_kernel<G, KIND, 256, 1>;
-            case 512: return decode_kernel<G, KIND, 512, 1>;
-            default: return nullptr;
-            }
-        }
-        if (depth == 2) {
-            switch (block) {
-            case 128: return decode_kernel<G, KIND, 128, 2>;
-            case 256: return decode_kernel<G, KIND, 256, 2>;
-            case 512: return decode_kernel<G, KIND, 512, 2>;
-            default: return nullptr;
-            }
-        }
-        if (depth == 3) {
-            switch (block) {
-            case 256: return decode_kernel<G, KIND, 256, 3>;
-            case 512: return decode_kernel<G, KIND, 512, 3>;
-            default: return nullptr;
-            }
-        }
-        return nullptr;
+template <int KIND, int G, int BLOCK> DecodeFn pick_depth(std::uint32_t depth) {
+    switch (depth) {
+    case 1: return decode_kernel<G, KIND, BLOCK, 1>;
+    case 2: return decode_kernel<G, KIND, BLOCK, 2>;
+    case 3: return decode_kernel<G, KIND, BLOCK, 3>;
+    default: return nullptr;
     }
 }
 
-template <int KIND> DecodeFn pick(std::uint32_t group, std::uint32_t block, std::uint32_t depth) {
-    switch (group) {
-    case 1: return pick_block<KIND, 1>(block, depth);
-    case 2: return pick_block<KIND, 2>(block, depth);
-    case 4: return pick_block<KIND, 4>(block, depth);
-    case 8: return pick_block<KIND, 8>(block, depth);
-    case 16: return pick_block<KIND, 16>(block, depth);
+template <int KIND, int G> DecodeFn pick_block(std::uint32_t block, std::uint32_t depth) {
+    switch (block) {
+    case 64: return pick_depth<KIND, G, 64>(depth);
+    case 128: return pick_depth<KIND, G, 128>(depth);
+    case 256: return pick_depth<KIND, G, 256>(depth);
+    case 512: return pick_depth<KIND, G, 512>(depth);
     default: return nullptr;
     }
 }
 
 } // namespace tsk
 
-// One translation unit per kind instantiates its kernels and exports this lookup.
-#define TSK_DEFINE_DECODE_TABLE(NAME, KIND)                                                          \
-    extern "C" tsk::DecodeFn NAME(std::uint32_t group, std::uint32_t block, std::uint32_t depth) { \
-        return tsk::pick<KIND>(group, block, depth);                                               \
+// One translation unit per kind and lane-group width instantiates its kernels
+// and exports their lookup by workgroup size and depth: decode_<kind>.hip,
+// compiled once for each width with -DTSK_DECODE_GROUP=1, 2, 4, 8 and 16, so
+// the builds of the 75 kernels of a kind run side by side.
+#define TSK_TABLE_NAME2(KIND_NAME, G) tsk_decode_table_##KIND_NAME##_g##G
+#define TSK_TABLE_NAME(KIND_NAME, G) TSK_TABLE_NAME2(KIND_NAME, G)
+#define TSK_DEFINE_DECODE_TABLE(KIND_NAME, KIND)                                                             \
+    extern "C" tsk::DecodeFn TSK_TABLE_NAME(KIND_NAME, TSK_DECODE_GROUP)(std::uint32_t block, std::uint32_t depth) { \
+        return tsk::pick_block<KIND, TSK_DECODE_GROUP>(block, depth);                                         \
     }

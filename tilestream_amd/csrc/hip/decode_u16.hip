@@ -1,4 +1,9 @@
 // Decode kernels of kind TSK_KIND_U16 (see decode_core.h).
+// Compiled once per lane-group width (-DTSK_DECODE_GROUP=1, 2, 4, 8, 16).
 #include "decode_core.h"
 
-TSK_DEFINE_DECODE_TABLE(tsk_decode_table_u16, TSK_KIND_U16)
+#ifndef TSK_DECODE_GROUP
+#error "decode_u16.hip is compiled with -DTSK_DECODE_GROUP set to 1, 2, 4, 8 or 16"
+#endif
+
+TSK_DEFINE_DECODE_TABLE(u16, TSK_KIND_U16)
