@@ -181,7 +181,7 @@ __device__ inline std::uint32_t payload_base(const tsk_block_tile &bt) {
 
 // Decodes and stores every region row of the staged block (any tile modes,
 // partial tiles, region edges).
-template <bool U16, bool CAL, int BLOCK, bool NT>
+template <bool U16, bool CAL, int BLOCK, bool NT, bool MAPS>
 __device__ inline void block_rows(const tsk_decode_image &img, const BlockJob &j, const tsk_block &b,
                                   const std::uint32_t *stage, const std::uint16_t *rows) {
     constexpr int G = (U16 && !CAL) ? 8 : 4;
@@ -230,24 +230,24 @@ __device__ inline void block_rows(const tsk_decode_image &img, const BlockJob &j
 #pragma unroll
             for (int k = 0; k < G; ++k) v.v[k] = payload[0];
         }
-        if (col0 < r.right && col0 + G > r.left) emit<G, U16, CAL, NT>(img, r, col0, v);
+        if (col0 < r.right && col0 + G > r.left) emit<G, U16, CAL, NT, MAPS>(img, r, col0, v);
     }
 }
 
 // Stores a group of G samples (or calibrated values) at a 16-byte aligned
-// output address: one 16-byte store.
-template <int G, bool U16, bool CAL, bool NT>
+// output address: one 16-byte store.  MAPS: the image has calibration maps.
+template <int G, bool U16, bool CAL, bool NT, bool MAPS>
 __device__ inline void store_group(const tsk_decode_image &img, gptr<std::uint8_t> line, std::uint64_t ex0,
                                    std::uint64_t ey, const Group<G> &g) {
     if constexpr (CAL) {
         float o[G], gn[G];
-        if (img.offset_map != nullptr) {
+        if (MAPS && img.offset_map != nullptr) {
             load_floats<G>(reinterpret_cast<gptr<const float>>(global(img.offset_map) + ey * img.offset_pitch) + ex0, o);
         } else {
 #pragma unroll
             for (int k = 0; k < G; ++k) o[k] = img.offset;
         }
-        if (img.gain_map != nullptr) {
+        if (MAPS && img.gain_map != nullptr) {
             load_floats<G>(reinterpret_cast<gptr<const float>>(global(img.gain_map) + ey * img.gain_pitch) + ex0, gn);
         } else {
 #pragma unroll
@@ -274,7 +274,7 @@ __device__ inline void store_group(const tsk_decode_image &img, gptr<std::uint8_
 // covers, with no mode, edge or region tests.  S (lanes per tile row) is a
 // constant, so the segment scan has no branches and the unrolled rows are
 // straight-line code whose LDS reads the compiler can overlap.
-template <bool U16, bool CAL, int BLOCK, bool NT, std::uint32_t S>
+template <bool U16, bool CAL, int BLOCK, bool NT, bool MAPS, std::uint32_t S>
 __device__ inline void block_rows_full(const tsk_decode_image &img, const BlockJob &j, const tsk_block &b,
                                        const std::uint32_t *stage, const std::uint16_t *rows) {
     constexpr int G = (U16 && !CAL) ? 8 : 4;
@@ -301,7 +301,7 @@ __device__ inline void block_rows_full(const tsk_decode_image &img, const BlockJ
         Group<G> v;
         std::uint32_t incl;
         predict_group<G, U16>(rb, col0, p, S, rb.seed, v, incl);
-        store_group<G, U16, CAL, NT>(img, line, ex0, ey0 + y, v);
+        store_group<G, U16, CAL, NT, MAPS>(img, line, ex0, ey0 + y, v);
         line += line_step;
     }
 }
@@ -321,58 +321,77 @@ __device__ inline bool block_is_full(const tsk_decode_image &img, const BlockJob
     return first % 16 == 0 && img.out_pitch % 16 == 0;
 }
 
-template <bool U16, bool CAL, int BLOCK>
-__device__ inline void block_rows_any(const tsk_decode_image &img, const BlockJob &j, const tsk_block &b,
-                                      const std::uint32_t *stage, const std::uint16_t *rows) {
+template <bool U16, bool CAL, int BLOCK, bool MAPS>
+__device__ inline __attribute__((always_inline)) void block_rows_as(const tsk_decode_image &img, const BlockJob &j, const tsk_block &b,
+                                     const std::uint32_t *stage, const std::uint16_t *rows) {
     constexpr std::uint32_t G = (U16 && !CAL) ? 8 : 4;
     const bool nt = block_whole_lines(img, j);
     if (block_is_full(img, j)) {
         switch (img.tile_w / G) {
         case 8:
             if constexpr (G == 8) {
-                if (nt) block_rows_full<U16, CAL, BLOCK, true, 8>(img, j, b, stage, rows);
-                else block_rows_full<U16, CAL, BLOCK, false, 8>(img, j, b, stage, rows);
+                if (nt) block_rows_full<U16, CAL, BLOCK, true, MAPS, 8>(img, j, b, stage, rows);
+                else block_rows_full<U16, CAL, BLOCK, false, MAPS, 8>(img, j, b, stage, rows);
             }
             return;
         case 16:
-            if (nt) block_rows_full<U16, CAL, BLOCK, true, 16>(img, j, b, stage, rows);
-            else block_rows_full<U16, CAL, BLOCK, false, 16>(img, j, b, stage, rows);
+            if (nt) block_rows_full<U16, CAL, BLOCK, true, MAPS, 16>(img, j, b, stage, rows);
+            else block_rows_full<U16, CAL, BLOCK, false, MAPS, 16>(img, j, b, stage, rows);
             return;
         case 32:
-            if (nt) block_rows_full<U16, CAL, BLOCK, true, 32>(img, j, b, stage, rows);
-            else block_rows_full<U16, CAL, BLOCK, false, 32>(img, j, b, stage, rows);
+            if (nt) block_rows_full<U16, CAL, BLOCK, true, MAPS, 32>(img, j, b, stage, rows);
+            else block_rows_full<U16, CAL, BLOCK, false, MAPS, 32>(img, j, b, stage, rows);
             return;
         default:
             if constexpr (G == 4) {
-                if (nt) block_rows_full<U16, CAL, BLOCK, true, 64>(img, j, b, stage, rows);
-                else block_rows_full<U16, CAL, BLOCK, false, 64>(img, j, b, stage, rows);
+                if (nt) block_rows_full<U16, CAL, BLOCK, true, MAPS, 64>(img, j, b, stage, rows);
+                else block_rows_full<U16, CAL, BLOCK, false, MAPS, 64>(img, j, b, stage, rows);
             }
             return;
         }
     } else if (nt) {
-        block_rows<U16, CAL, BLOCK, true>(img, j, b, stage, rows);
+        block_rows<U16, CAL, BLOCK, true, MAPS>(img, j, b, stage, rows);
     } else {
-        block_rows<U16, CAL, BLOCK, false>(img, j, b, stage, rows);
+        block_rows<U16, CAL, BLOCK, false, MAPS>(img, j, b, stage, rows);
+    }
+}
+
+// SPLIT: calibrated images with maps and without take separate paths (chosen
+// per block), so that the store loops of an image without maps issue no
+// loads (a wait for a load also waits for every store issued before it).
+// The mixed kind keeps one path, which tests for maps per group: its kernel
+// holds every pixel type and calibration, and a second copy of the
+// calibrated paths would push it past its register budget.
+template <bool U16, bool CAL, int BLOCK, bool SPLIT>
+__device__ inline __attribute__((always_inline)) void block_rows_any(const tsk_decode_image &img, const BlockJob &j,
+                                                                     const tsk_block &b, const std::uint32_t *stage,
+                                                                     const std::uint16_t *rows) {
+    if constexpr (CAL && SPLIT) {
+        if (has_maps(img)) block_rows_as<U16, CAL, BLOCK, true>(img, j, b, stage, rows);
+        else block_rows_as<U16, CAL, BLOCK, false>(img, j, b, stage, rows);
+    } else {
+        block_rows_as<U16, CAL, BLOCK, CAL>(img, j, b, stage, rows);
     }
 }
 
 template <int KIND, int BLOCK>
-__device__ inline void block_decode(const tsk_decode_image &img, const BlockJob &j, const tsk_block &b,
-                                    const std::uint32_t *stage, const std::uint16_t *rows) {
+__device__ inline __attribute__((always_inline)) void block_decode(const tsk_decode_image &img, const BlockJob &j,
+                                                                   const tsk_block &b, const std::uint32_t *stage,
+                                                                   const std::uint16_t *rows) {
     if constexpr (KIND == TSK_KIND_U16) {
-        block_rows_any<true, false, BLOCK>(img, j, b, stage, rows);
+        block_rows_any<true, false, BLOCK, true>(img, j, b, stage, rows);
     } else if constexpr (KIND == TSK_KIND_F32) {
-        block_rows_any<false, false, BLOCK>(img, j, b, stage, rows);
+        block_rows_any<false, false, BLOCK, true>(img, j, b, stage, rows);
     } else if constexpr (KIND == TSK_KIND_U16_CAL) {
-        block_rows_any<true, true, BLOCK>(img, j, b, stage, rows);
+        block_rows_any<true, true, BLOCK, true>(img, j, b, stage, rows);
     } else if constexpr (KIND == TSK_KIND_F32_CAL) {
-        block_rows_any<false, true, BLOCK>(img, j, b, stage, rows);
+        block_rows_any<false, true, BLOCK, true>(img, j, b, stage, rows);
     } else if (img.type == TSK_UINT16) {
-        if (img.calibrated) block_rows_any<true, true, BLOCK>(img, j, b, stage, rows);
-        else block_rows_any<true, false, BLOCK>(img, j, b, stage, rows);
+        if (img.calibrated) block_rows_any<true, true, BLOCK, false>(img, j, b, stage, rows);
+        else block_rows_any<true, false, BLOCK, false>(img, j, b, stage, rows);
     } else {
-        if (img.calibrated) block_rows_any<false, true, BLOCK>(img, j, b, stage, rows);
-        else block_rows_any<false, false, BLOCK>(img, j, b, stage, rows);
+        if (img.calibrated) block_rows_any<false, true, BLOCK, false>(img, j, b, stage, rows);
+        else block_rows_any<false, false, BLOCK, false>(img, j, b, stage, rows);
     }
 }
 

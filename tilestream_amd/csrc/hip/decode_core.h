@@ -33,13 +33,20 @@
 
 namespace tsk {
 
-// Largest payload (64 x 64 float32 RAW) in words; with up to 3 words of 16-byte
-// misalignment ahead of it, at most 1025 16-byte blocks are staged.  The slack
-// covers the farthest read past a row end: a lane of the last segment reads up
-// to (G S - 1) b / 32 + 1 <= 256 words beyond the row's first packed word.
+// Largest payload in words: a tile's payload is at most its raw size (the
+// parser and the device-side index checks reject longer ones), 4096 words for
+// a 4096-sample float32 tile and 2048 for a uint16 one.  With up to 3 words of
+// 16-byte misalignment ahead of it, at most 1025 (float32) or 513 (uint16)
+// 16-byte blocks are staged.  The slack covers the farthest read past a row
+// end: a lane of the last segment reads up to (G S - 1) b / 32 + 1 <= 256
+// words beyond the row's first packed word.  Kernels of the uint16 kinds use
+// the smaller stage, so twice as many of their workgroups fit in a CU's LDS.
+constexpr std::uint32_t stage_blocks_for(std::uint32_t payload_words) { return (payload_words + 3 + 3) / 4; }
 constexpr std::uint32_t kPayloadWords = 4096;
-constexpr std::uint32_t kMaxBlocks = (kPayloadWords + 3 + 3) / 4;
-constexpr std::uint32_t kStageWords = kMaxBlocks * 4 + 280;
+template <int KIND> constexpr std::uint32_t stage_blocks() {
+    return stage_blocks_for(KIND == TSK_KIND_U16 || KIND == TSK_KIND_U16_CAL ? kPayloadWords / 2 : kPayloadWords);
+}
+template <int KIND> constexpr std::uint32_t stage_words() { return stage_blocks<KIND>() * 4 + 280; }
 
 // The descriptor tables never change while a decode runs.  Reading them through
 // the constant address space lets the compiler use scalar loads (the scalar
@@ -203,14 +210,40 @@ template <int G> __device__ inline void load_floats(gptr<const float> src, float
     }
 }
 
+// Whether an image's calibration reads per-pixel maps (else both operands
+// are scalars).
+__device__ inline bool has_maps(const tsk_decode_image &img) {
+    return img.offset_map != nullptr || img.gain_map != nullptr;
+}
+
 // Stores G samples (or their calibrated values) for tile columns col0..col0+G-1.
 // Groups wholly inside the region store whole runs; groups on its left or
-// right edge store only the region's columns.
-template <int G, bool U16, bool CAL, bool NT = false>
+// right edge store only the region's columns.  MAPS: the image may have
+// per-pixel calibration maps (tested per group); without, the operands are
+// the scalars.  Kernels choose MAPS per image outside their store loops, so
+// that the stores of an image without maps issue no loads: a wait for a load
+// also waits for every store issued before it.
+template <int G, bool U16, bool CAL, bool NT, bool MAPS>
 __device__ inline void emit(const tsk_decode_image &img, const Region &r, std::uint32_t col0, const Group<G> &g) {
     const bool inside = col0 >= r.left && col0 + G <= r.right;
     const std::int64_t ex0 = r.ex_tile + col0;
-    if constexpr (CAL) {
+    if constexpr (CAL && !MAPS) {
+        gptr<float> dst = reinterpret_cast<gptr<float>>(r.line) + ex0;
+        if (inside) {
+            std::uint32_t out[G];
+#pragma unroll
+            for (int k = 0; k < G; ++k)
+                // Two roundings to nearest even, never contracted into an FMA.
+                out[k] = __float_as_uint(__fmul_rn(__fsub_rn(sample_float(g.v[k], U16), img.offset), img.gain));
+            store_words<G, NT>(reinterpret_cast<gptr<std::uint32_t>>(dst), out);
+        } else {
+#pragma unroll
+            for (int k = 0; k < G; ++k) {
+                const std::uint32_t c = col0 + k;
+                if (c >= r.left && c < r.right) dst[k] = __fmul_rn(__fsub_rn(sample_float(g.v[k], U16), img.offset), img.gain);
+            }
+        }
+    } else if constexpr (CAL) {
         gptr<float> dst = reinterpret_cast<gptr<float>>(r.line) + ex0;
         const bool omap = img.offset_map != nullptr, gmap = img.gain_map != nullptr;
         const gptr<const float> orow =
@@ -234,7 +267,6 @@ __device__ inline void emit(const tsk_decode_image &img, const Region &r, std::u
             std::uint32_t out[G];
 #pragma unroll
             for (int k = 0; k < G; ++k)
-                // Two roundings to nearest even, never contracted into an FMA.
                 out[k] = __float_as_uint(__fmul_rn(__fsub_rn(sample_float(g.v[k], U16), o[k]), gn[k]));
             store_words<G, NT>(reinterpret_cast<gptr<std::uint32_t>>(dst), out);
         } else {
@@ -407,7 +439,7 @@ __device__ inline bool whole_lines(const tsk_decode_image &img, const TileView &
 }
 
 // Decodes the staged tile and writes its region pixels as it goes.
-template <int G, bool U16, bool CAL, int BLOCK>
+template <int G, bool U16, bool CAL, bool MAPS, int BLOCK>
 __device__ void decode_tile(const tsk_decode_image &img, const TileView &t, const std::uint32_t *stage,
                             const std::uint16_t *row_start) {
     const std::uint32_t S = (img.tile_w / G) < 64u ? img.tile_w / G : 64u;
@@ -429,14 +461,14 @@ __device__ void decode_tile(const tsk_decode_image &img, const TileView &t, cons
                 if (passes > 1) // S = 64: lane 63 holds the row total so far
                     carry = Ring<U16>::add(carry, static_cast<std::uint32_t>(
                                                       __builtin_amdgcn_readlane(static_cast<int>(incl), 63)));
-                emit<G, U16, CAL>(img, r, col0, v);
+                emit<G, U16, CAL, false, MAPS>(img, r, col0, v);
             }
         } else {
             for (std::uint32_t pass = 0; pass < passes; ++pass) {
                 const std::uint32_t col0 = (pass * S + p) * G;
                 Group<G> v;
                 stored_group<G, U16>(t, payload, row, col0, v);
-                emit<G, U16, CAL>(img, r, col0, v);
+                emit<G, U16, CAL, false, MAPS>(img, r, col0, v);
             }
         }
     }
@@ -482,20 +514,35 @@ __device__ inline void decode_tile_held(const tsk_decode_image &img, const TileV
     }
 }
 
-template <int G, bool U16, bool CAL, int BLOCK, int K, bool NT>
+template <int G, bool U16, bool CAL, int BLOCK, int K, bool NT, bool MAPS>
 __device__ inline void held_store_as(const tsk_decode_image &img, const TileView &t, const Held<G, K> &h) {
     const std::uint32_t col0 = (threadIdx.x % (img.tile_w / G)) * G;
 #pragma unroll
     for (int k = 0; k < K; ++k)
-        if (h.row[k] < t.bottom) emit<G, U16, CAL, NT>(img, region_of(img, t, h.row[k]), col0, h.v[k]);
+        if (h.row[k] < t.bottom) emit<G, U16, CAL, NT, MAPS>(img, region_of(img, t, h.row[k]), col0, h.v[k]);
 }
 
 // Tiles whose output rows span whole lines store them non-temporally (where a
-// group is one store instruction).
-template <int G, bool U16, bool CAL, int BLOCK, int K>
+// group is one store instruction).  SPLIT: calibrated images with maps and
+// without take separate paths, so that the stores of an image without maps
+// issue no loads (a wait for a load also waits for every store issued before
+// it).  The mixed kind keeps one path, which tests for maps per group: a
+// second copy of its calibrated paths costs its plain paths registers.
+template <int G, bool U16, bool CAL, int BLOCK, int K, bool SPLIT>
 __device__ inline void held_store(const tsk_decode_image &img, const TileView &t, const Held<G, K> &h) {
-    if (whole_lines(img, t)) held_store_as<G, U16, CAL, BLOCK, K, true>(img, t, h);
-    else held_store_as<G, U16, CAL, BLOCK, K, false>(img, t, h);
+    const bool nt = whole_lines(img, t);
+    if constexpr (CAL && SPLIT) {
+        if (has_maps(img)) {
+            if (nt) held_store_as<G, U16, CAL, BLOCK, K, true, true>(img, t, h);
+            else held_store_as<G, U16, CAL, BLOCK, K, false, true>(img, t, h);
+            return;
+        }
+        if (nt) held_store_as<G, U16, CAL, BLOCK, K, true, false>(img, t, h);
+        else held_store_as<G, U16, CAL, BLOCK, K, false, false>(img, t, h);
+    } else {
+        if (nt) held_store_as<G, U16, CAL, BLOCK, K, true, CAL>(img, t, h);
+        else held_store_as<G, U16, CAL, BLOCK, K, false, CAL>(img, t, h);
+    }
 }
 
 // A tile whose payload is being fetched.
@@ -591,6 +638,17 @@ __device__ inline Pending describe(const tsk_decode_args &a, std::uint64_t i) {
     }
     const tsk_decode_task task = load_constant(a.tasks, i);
     return describe_task(a, task, task.image);
+}
+
+// describe() for a kernel of kind KIND: a payload longer than the kind's
+// stage (which the checks above already exclude) is malformed, never staged.
+template <int KIND> __device__ inline Pending describe_for(const tsk_decode_args &a, std::uint64_t i) {
+    Pending p = describe(a, i);
+    if (p.nblocks > stage_blocks<KIND>()) {
+        p.state = 2;
+        p.nblocks = 0;
+    }
+    return p;
 }
 
 // Work assignment.  Without a work counter: a static stride over the tasks.
@@ -750,27 +808,31 @@ __device__ inline void prefetch(const tsk_decode_args &a, const Pending &p, u32x
 template <int G, int KIND, int BLOCK>
 __device__ inline void decode_tile_as(const tsk_decode_image &img, const TileView &t, const std::uint32_t *stage,
                                       const std::uint16_t *row_start) {
+    const bool maps = has_maps(img);
     if constexpr (KIND == TSK_KIND_U16) {
-        decode_tile<G, true, false, BLOCK>(img, t, stage, row_start);
+        decode_tile<G, true, false, false, BLOCK>(img, t, stage, row_start);
     } else if constexpr (KIND == TSK_KIND_F32) {
-        decode_tile<G, false, false, BLOCK>(img, t, stage, row_start);
+        decode_tile<G, false, false, false, BLOCK>(img, t, stage, row_start);
     } else if constexpr (KIND == TSK_KIND_U16_CAL) {
-        decode_tile<G, true, true, BLOCK>(img, t, stage, row_start);
+        if (maps) decode_tile<G, true, true, true, BLOCK>(img, t, stage, row_start);
+        else decode_tile<G, true, true, false, BLOCK>(img, t, stage, row_start);
     } else if constexpr (KIND == TSK_KIND_F32_CAL) {
-        decode_tile<G, false, true, BLOCK>(img, t, stage, row_start);
+        if (maps) decode_tile<G, false, true, true, BLOCK>(img, t, stage, row_start);
+        else decode_tile<G, false, true, false, BLOCK>(img, t, stage, row_start);
     } else if (img.type == TSK_UINT16) {
-        if (img.calibrated) decode_tile<G, true, true, BLOCK>(img, t, stage, row_start);
-        else decode_tile<G, true, false, BLOCK>(img, t, stage, row_start);
+        // The mixed kind: one calibrated path that tests for maps per group.
+        if (img.calibrated) decode_tile<G, true, true, true, BLOCK>(img, t, stage, row_start);
+        else decode_tile<G, true, false, false, BLOCK>(img, t, stage, row_start);
     } else {
-        if (img.calibrated) decode_tile<G, false, true, BLOCK>(img, t, stage, row_start);
-        else decode_tile<G, false, false, BLOCK>(img, t, stage, row_start);
+        if (img.calibrated) decode_tile<G, false, true, true, BLOCK>(img, t, stage, row_start);
+        else decode_tile<G, false, false, false, BLOCK>(img, t, stage, row_start);
     }
 }
 
 // Depth 1: the payload of a tile is loaded after the previous one is decoded.
 template <int G, int KIND, int BLOCK>
 __device__ inline __attribute__((always_inline)) void decode_body_staged(const tsk_decode_args &a) {
-    __shared__ __attribute__((aligned(16))) std::uint32_t stage[kStageWords];
+    __shared__ __attribute__((aligned(16))) std::uint32_t stage[stage_words<KIND>()];
     __shared__ std::uint16_t row_start[64];
     __shared__ std::uint32_t shared_flag;
     __shared__ std::uint64_t shared_next[1];
@@ -781,7 +843,7 @@ __device__ inline __attribute__((always_inline)) void decode_body_staged(const t
     for (;;) {
         const std::uint64_t i = draw<BLOCK>(a, cursor, shared_next);
         if (i >= a.task_count) return;
-        const Pending c = describe(a, i);
+        const Pending c = describe_for<KIND>(a, i);
         __syncthreads(); // the previous tile is done with the LDS
         for (std::uint32_t k = tid; k < c.nblocks; k += BLOCK) *reinterpret_cast<u32x4 *>(stage + 4 * k) = c.blocks[k];
         if (c.mode == 2 && c.rows != TSK_NO_ROWS && tid < c.h && c.rows + tid < a.row_count)
@@ -842,19 +904,19 @@ template <int G, int KIND, int BLOCK, int K>
 __device__ inline __attribute__((always_inline)) void store_phase(const tsk_decode_image &img, const TileView &t,
                                                                   const Held<G, K> &h) {
     if constexpr (KIND == TSK_KIND_U16) {
-        held_store<G, true, false, BLOCK, K>(img, t, h);
+        held_store<G, true, false, BLOCK, K, true>(img, t, h);
     } else if constexpr (KIND == TSK_KIND_F32) {
-        held_store<G, false, false, BLOCK, K>(img, t, h);
+        held_store<G, false, false, BLOCK, K, true>(img, t, h);
     } else if constexpr (KIND == TSK_KIND_U16_CAL) {
-        held_store<G, true, true, BLOCK, K>(img, t, h);
+        held_store<G, true, true, BLOCK, K, true>(img, t, h);
     } else if constexpr (KIND == TSK_KIND_F32_CAL) {
-        held_store<G, false, true, BLOCK, K>(img, t, h);
+        held_store<G, false, true, BLOCK, K, true>(img, t, h);
     } else if (img.type == TSK_UINT16) {
-        if (img.calibrated) held_store<G, true, true, BLOCK, K>(img, t, h);
-        else held_store<G, true, false, BLOCK, K>(img, t, h);
+        if (img.calibrated) held_store<G, true, true, BLOCK, K, false>(img, t, h);
+        else held_store<G, true, false, BLOCK, K, false>(img, t, h);
     } else {
-        if (img.calibrated) held_store<G, false, true, BLOCK, K>(img, t, h);
-        else held_store<G, false, false, BLOCK, K>(img, t, h);
+        if (img.calibrated) held_store<G, false, true, BLOCK, K, false>(img, t, h);
+        else held_store<G, false, false, BLOCK, K, false>(img, t, h);
     }
 }
 
@@ -878,12 +940,12 @@ __device__ inline void commit(const tsk_decode_args &a, const Pending &q, const 
 // round.
 template <int G, int KIND, int BLOCK, int DEPTH>
 struct Pipeline {
-    static constexpr int KMAX = static_cast<int>((kMaxBlocks + BLOCK - 1) / BLOCK);
+    static constexpr int KMAX = static_cast<int>((stage_blocks<KIND>() + BLOCK - 1) / BLOCK);
     static constexpr int K = held_groups<G, BLOCK>();
     static constexpr int SETS = DEPTH - 1;
 
     const tsk_decode_args &a;
-    std::uint32_t (*stage)[kStageWords];
+    std::uint32_t (*stage)[stage_words<KIND>()];
     std::uint16_t (*row_start)[64];
     std::uint32_t *shared_flag;
     std::uint64_t *shared_next;
@@ -898,7 +960,7 @@ struct Pipeline {
     __device__ inline __attribute__((always_inline)) void load(int set, std::uint32_t slot) {
         const std::uint64_t n = draw<BLOCK>(a, cursor, shared_next);
         more[slot] = n < a.task_count;
-        if (more[slot]) p[slot] = describe(a, n);
+        if (more[slot]) p[slot] = describe_for<KIND>(a, n);
         else p[slot].nblocks = 0, p[slot].mode = 0;
         prefetch<BLOCK>(a, p[slot], reg[set], row_reg[set]);
     }
@@ -935,7 +997,7 @@ struct Pipeline {
         if (a.queue != nullptr) queue_start(a, cursor);
         const std::uint64_t i = draw<BLOCK>(a, cursor, shared_next);
         if (i >= a.task_count) return;
-        c = describe(a, i);
+        c = describe_for<KIND>(a, i);
         prefetch<BLOCK>(a, c, reg[0], row_reg[0]);
         await_prefetch(reg[0], row_reg[0], cursor.ahead);
         commit<BLOCK>(a, c, reg[0], row_reg[0], stage[0], row_start[0]);
@@ -959,7 +1021,7 @@ struct Pipeline {
 
 template <int G, int KIND, int BLOCK, int DEPTH>
 __device__ inline __attribute__((always_inline)) void decode_body_prefetch(const tsk_decode_args &a) {
-    __shared__ __attribute__((aligned(16))) std::uint32_t stage[2][kStageWords];
+    __shared__ __attribute__((aligned(16))) std::uint32_t stage[2][stage_words<KIND>()];
     __shared__ std::uint16_t row_start[2][64];
     __shared__ std::uint32_t shared_flag;
     __shared__ std::uint64_t shared_next[1];
