@@ -108,6 +108,12 @@ struct ServerOptions {
     // large: it is batched only with large requests, and large batches hold at
     // most streams - 1 streams, so small requests never wait behind a large one.
     std::uint64_t large_bytes = 32ull << 20;
+    // A batch of region requests with at least this many tiles, every request
+    // on a device-resident archive, is decoded by the block decoder (up to 8
+    // neighbouring tiles per unit, block records built on the device); 0:
+    // always tile by tile.  Smaller batches keep the tile decoder, whose
+    // single launch has the lower latency.
+    std::uint64_t block_tasks = 16384;
 };
 
 struct ServerStats {

@@ -213,6 +213,7 @@ typedef struct {
     uint64_t staging_bytes;        /* per stream, device staging for tiles of host archives */
     ts_launch_shape shape;
     uint64_t large_bytes;          /* requests with this many output bytes batch apart, on streams - 1 at most */
+    uint64_t block_tasks;          /* region batches of this many tiles (device archives) decode by blocks; 0: never */
 } ts_server_options;
 typedef struct {
     uint64_t requests, batches, decoded_bytes, failed;

@@ -78,14 +78,19 @@ typedef struct {
  * covers tasks [first_task, next request's first_task): the tiles of columns
  * col0.. col0+cols-1 and rows row0.., row-major, of the archive whose tiles
  * start at tile_base in the resident tile table (tsk_decode_args.tasks).
- * tsk_decode_args.request_index holds the request of every task. */
+ * tsk_decode_args.request_index holds the request of every task.  For the
+ * block decoder (tsk_build_blocks) request r also covers blocks
+ * [first_block, next request's first_block): its tile grid cut into blocks
+ * of across x down tiles, row-major. */
 typedef struct {
     uint64_t first_task;
     uint32_t tile_base;          /* index of the archive's tile 0 in the tile table */
     uint32_t tile_columns;       /* tile columns of the archive */
     uint32_t col0, row0;         /* first tile of the region */
     uint32_t cols;               /* tiles per region row */
-    uint32_t reserved;
+    uint32_t rows;               /* tile rows of the region */
+    uint32_t first_block;        /* block decoding: the request's first block */
+    uint16_t across, down;       /* block decoding: tiles of a full block */
 } tsk_request;
 
 /* Block decode: a block is up to 8 tiles of one image, `across` x `down`
@@ -125,6 +130,18 @@ typedef struct {
 /* Every tile of the block is predictive and full size, and the block has
  * its full number of tiles both ways. */
 #define TSK_BLOCK_FULL 1u
+
+/* Block records of a batch of region requests, built on the device from the
+ * requests and the resident tile table (one record per block, in request
+ * order): the block decoder's input for request batches. */
+typedef struct {
+    const tsk_request *requests;
+    const uint32_t *block_index;     /* block -> request */
+    const tsk_decode_image *images;  /* one per request */
+    const tsk_decode_task *tiles;    /* the resident tile table */
+    tsk_block *blocks;               /* out: block_count records */
+    uint64_t block_count;
+} tsk_block_build_args;
 
 typedef struct {
     const tsk_decode_image *images;
@@ -259,15 +276,20 @@ int tsk_index_tiles(const tsk_index_args *args, void *stream);
 int tsk_results_begin(void *results, uint32_t count, uint32_t *counter, uint32_t words, void *stream);
 int tsk_results_complete(const tsk_decode_image *images, uint32_t count, void *results,
                          void *stream);
-/* tsk_results_complete that writes the final records to `out` (for example
- * mapped host memory) and leaves `results` as the decode left them. */
 /* Request mode: copies the 16-byte blocks holding each staged task's payload
  * from where the resident tile table points (mapped host memory) into
  * `staging` (args->staged, args->staging set). */
 int tsk_stage_payloads(const tsk_decode_args *args, void *stream);
+/* tsk_results_complete that writes the final records to `out` (for example
+ * mapped host memory) and leaves `results` as the decode left them. */
 int tsk_results_publish(const tsk_decode_image *images, uint32_t count, const void *results, void *out,
                         void *stream);
 int tsk_decode(const tsk_decode_args *args, void *stream);
+/* Builds the block records of a batch of region requests (one wave per
+ * block).  Every request's archive is device-resident, and a block of its
+ * geometry stages at most TSK_BLOCK_STAGE_USED 16-byte blocks even with every
+ * tile RAW (the caller checks it). */
+int tsk_build_blocks(const tsk_block_build_args *args, void *stream);
 /* Resident workgroups per CU of a decode kernel variant (for persistent grids). */
 int tsk_decode_occupancy(uint32_t kind, uint32_t group, uint32_t block, uint32_t depth, int *blocks);
 /* Whether a (group, block, depth) combination has a compiled kernel. */

@@ -307,6 +307,54 @@ int tsk_decode(const tsk_decode_args *args, void *) {
     return hipSuccess;
 }
 
+// The block records of a request batch, built as the device kernel builds
+// them, with every table index checked.
+int tsk_build_blocks(const tsk_block_build_args *args, void *) {
+    const tsk_block_build_args &a = *args;
+    check(inside_allocation(a.blocks, a.block_count * sizeof(tsk_block)), "block records");
+    check(inside_allocation(a.block_index, a.block_count * sizeof(std::uint32_t)), "block index");
+    for (std::uint64_t b = 0; b < a.block_count; ++b) {
+        const std::uint32_t r = a.block_index[b];
+        check(inside_allocation(a.requests + r, sizeof(tsk_request)), "block request");
+        const tsk_request &q = a.requests[r];
+        check(q.across >= 1 && q.down >= 1 && std::uint32_t{q.across} * q.down <= 8 && b >= q.first_block, "block grid");
+        const std::uint32_t local = static_cast<std::uint32_t>(b - q.first_block);
+        const std::uint32_t nbx = (q.cols + q.across - 1) / q.across;
+        const std::uint32_t by = local / nbx, bx = local - by * nbx;
+        check(by * q.down < q.rows, "block row");
+        const std::uint32_t across = std::min<std::uint32_t>(q.across, q.cols - bx * q.across);
+        const std::uint32_t down = std::min<std::uint32_t>(q.down, q.rows - by * q.down);
+        const tsk_decode_image &img = a.images[r];
+        tsk_block rec{};
+        rec.image = r;
+        rec.across = static_cast<std::uint16_t>(across);
+        rec.down = static_cast<std::uint16_t>(down);
+        rec.col0 = q.col0 + bx * q.across;
+        rec.row0 = q.row0 + by * q.down;
+        rec.count = across * down;
+        bool full = across == q.across && down == q.down;
+        for (std::uint32_t k = 0; k < rec.count; ++k) {
+            const std::uint64_t at = std::uint64_t{q.tile_base} + std::uint64_t{rec.row0 + k / across} * q.tile_columns +
+                                     rec.col0 + k % across;
+            check(inside_allocation(a.tiles + at, sizeof(tsk_decode_task)), "block tile");
+            const tsk_decode_task &t = a.tiles[at];
+            const auto p = reinterpret_cast<std::uintptr_t>(t.payload);
+            tsk_block_tile &bt = rec.tiles[k];
+            bt.payload = t.payload;
+            bt.words = t.words;
+            bt.shape = t.shape;
+            bt.rows = t.rows;
+            bt.stage = rec.blocks;
+            rec.blocks += static_cast<std::uint32_t>(((p + std::uint64_t{t.words} * 4 + 15) >> 4) - (p >> 4));
+            full = full && t.shape == TSK_TASK_SHAPE(2u, img.tile_h, img.tile_w) && t.rows != TSK_NO_ROWS;
+        }
+        rec.flags = full ? TSK_BLOCK_FULL : 0u;
+        rec.image_copy = img;
+        a.blocks[b] = rec;
+    }
+    return hipSuccess;
+}
+
 int tsk_stage_payloads(const tsk_decode_args *args, void *) {
     check(args->staged != nullptr && args->staging != nullptr, "staging tables");
     return hipSuccess;

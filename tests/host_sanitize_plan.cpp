@@ -413,6 +413,14 @@ void serving(std::mt19937 &rng) {
     ServerStats stats;
     EXPECT(server->replay(requests.data(), arrival.data(), requests.size(), latency.data(), results.data(), stats) ==
            Status::success);
+    // The same requests with every region batch decoded by blocks (records
+    // built by the stand-in's block builder, which checks every table index).
+    options.block_tasks = 1;
+    std::unique_ptr<Server> by_blocks;
+    EXPECT(Server::create(*set, options, by_blocks) == Status::success);
+    EXPECT(by_blocks->run_batch(requests.data(), requests.size(), results.data()) == Status::success);
+    EXPECT(by_blocks->replay(requests.data(), arrival.data(), requests.size(), latency.data(), results.data(),
+                             stats) == Status::success);
     // A request past the archive's edge is rejected.
     ServeRequest bad = requests[0];
     bad.roi.x = archives[0].w;

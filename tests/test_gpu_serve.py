@@ -107,6 +107,37 @@ def test_batch_matches_oracle(orc, resident, shape):
         assert _same(out.read(specs, i), _expected(orc, resident, spec)), (shape, i, spec)
 
 
+@pytest.mark.parametrize("max_batch", [16, 1024])
+def test_block_batches_match_oracle(orc, resident, max_batch):
+    """Every region batch decoded by blocks (block records built on the device
+    from the requests): both pixel types, all three tile shapes, plain and
+    calibrated, regions cutting blocks on every side."""
+    rng = np.random.default_rng(610 + max_batch)
+    specs = _requests(resident, rng, 90)
+    out = Outputs(resident, specs)
+    server = api.Server(resident.set, api.server_options(streams=3, max_batch=max_batch, block_tasks=1))
+    st, statuses, _ = server.run_batch(out.requests)
+    assert st == ts.Status.success
+    assert (statuses == 0).all()
+    for i, spec in enumerate(specs):
+        assert _same(out.read(specs, i), _expected(orc, resident, spec)), (max_batch, i, spec)
+
+
+def test_blocks_and_tiles_give_the_same_words(resident):
+    """The same batch decoded by blocks and tile by tile, replayed with Poisson
+    arrivals: identical outputs word for word."""
+    rng = np.random.default_rng(613)
+    specs = _requests(resident, rng, 120, big_every=3)
+    by_tiles, by_blocks = Outputs(resident, specs), Outputs(resident, specs)
+    arrivals = np.cumsum(rng.exponential(2e4, len(specs))).astype(np.uint64)
+    for out, block_tasks in ((by_tiles, 0), (by_blocks, 1)):
+        server = api.Server(resident.set, api.server_options(streams=3, max_batch=32, block_tasks=block_tasks))
+        st, _, statuses, _, _ = server.replay(out.requests, arrivals)
+        assert st == ts.Status.success and (statuses == 0).all()
+    for i in range(len(specs)):
+        assert np.array_equal(by_tiles.read(specs, i), by_blocks.read(specs, i)), (i, specs[i])
+
+
 def test_many_single_tile_requests_in_one_batch(orc, resident):
     """Hundreds of one-tile requests share each sampled request-index stride."""
     rng = np.random.default_rng(77)

@@ -143,7 +143,7 @@ class ServeRequest(C.Structure):
 class ServerOptions(C.Structure):
     _fields_ = [("streams", C.c_uint32), ("max_batch", C.c_uint32), ("max_tasks", C.c_uint32),
                 ("encode_window", C.c_uint32), ("admission_bytes", C.c_uint64), ("staging_bytes", C.c_uint64),
-                ("shape", LaunchShape), ("large_bytes", C.c_uint64)]
+                ("shape", LaunchShape), ("large_bytes", C.c_uint64), ("block_tasks", C.c_uint64)]
 
 
 class TiffReadLimits(C.Structure):

@@ -638,13 +638,14 @@ class ArchiveSet:
 
 
 def server_options(streams=None, max_batch=None, max_tasks=None, admission_bytes=None, shape=None,
-                   encode_window=None, staging_bytes=None, large_bytes=None) -> ServerOptions:
+                   encode_window=None, staging_bytes=None, large_bytes=None, block_tasks=None) -> ServerOptions:
     """The library's default server options with the given fields replaced."""
     o = ServerOptions()
     N.lib().ts_server_default_options(C.byref(o))
     for name, value in (("streams", streams), ("max_batch", max_batch), ("max_tasks", max_tasks),
                         ("admission_bytes", admission_bytes), ("encode_window", encode_window),
-                        ("staging_bytes", staging_bytes), ("large_bytes", large_bytes)):
+                        ("staging_bytes", staging_bytes), ("large_bytes", large_bytes),
+                        ("block_tasks", block_tasks)):
         if value is not None:
             setattr(o, name, value)
     if shape is not None:

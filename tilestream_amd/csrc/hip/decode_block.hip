@@ -431,6 +431,86 @@ __global__ void __launch_bounds__(BLOCK, 2 * BLOCK / 256) block_decode_kernel(ts
     }
 }
 
+// Block records of a batch of region requests, one wave per block (four per
+// workgroup): the wave finds its request and its place in the request's
+// block grid, lane k reads tile k's entry of the resident tile table, the
+// stage offsets are the lanes' running sum of staged 16-byte blocks, and the
+// 104 words of the record are stored by consecutive lanes.  The same record
+// as the host's block planning (plan_blocks) makes for a planned batch.
+static_assert(offsetof(tsk_block, tiles) == 32 && offsetof(tsk_block, image_copy) == 288 &&
+                  sizeof(tsk_decode_image) == 120 && sizeof(tsk_block_tile) == 32,
+              "record words: 8 header, 8 per tile, 30 image, 2 pad");
+
+__global__ void __launch_bounds__(256) build_blocks_kernel(tsk_block_build_args a) {
+    const std::uint32_t lane = threadIdx.x & 63u;
+    const std::uint64_t b = std::uint64_t{blockIdx.x} * 4u + threadIdx.x / 64u;
+    if (b >= a.block_count) return; // uniform per wave
+    const std::uint32_t r = a.block_index[b];
+    const tsk_request q = a.requests[r];
+    const std::uint32_t qa = q.across, qd = q.down;
+    const std::uint32_t local = static_cast<std::uint32_t>(b - q.first_block);
+    const std::uint32_t nbx = (q.cols + qa - 1) / qa;
+    const std::uint32_t by = local / nbx, bx = local - by * nbx;
+    const std::uint32_t across = min(qa, q.cols - bx * qa), down = min(qd, q.rows - by * qd);
+    const std::uint32_t col0 = q.col0 + bx * qa, row0 = q.row0 + by * qd;
+    const std::uint32_t count = across * down;
+    const tsk_decode_image *img = a.images + r;
+    const std::uint32_t tile_w = img->tile_w, tile_h = img->tile_h;
+
+    // Tile `lane` of the block (lanes below count): its record words.
+    std::uint32_t f[6] = {0, 0, 0, 0, 0, 0}; // payload low, high, words, shape, rows, stage
+    std::uint32_t staged = 0;
+    bool full = true;
+    if (lane < count) {
+        const std::uint32_t kr = lane / across, kc = lane - kr * across;
+        const tsk_decode_task t =
+            a.tiles[std::uint64_t{q.tile_base} + std::uint64_t{row0 + kr} * q.tile_columns + col0 + kc];
+        const std::uintptr_t p = reinterpret_cast<std::uintptr_t>(t.payload);
+        staged = static_cast<std::uint32_t>(((p + std::uint64_t{t.words} * 4u + 15u) >> 4) - (p >> 4));
+        full = t.shape == TSK_TASK_SHAPE(2u, tile_h, tile_w) && t.rows != TSK_NO_ROWS;
+        f[0] = static_cast<std::uint32_t>(p);
+        f[1] = static_cast<std::uint32_t>(p >> 32);
+        f[2] = t.words;
+        f[3] = t.shape;
+        f[4] = t.rows;
+    }
+    std::uint32_t stage = 0, total = 0;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        const std::uint32_t v = static_cast<std::uint32_t>(__shfl(static_cast<int>(staged), k));
+        if (static_cast<std::uint32_t>(k) < lane) stage += v;
+        total += v;
+    }
+    f[5] = lane < count ? stage : 0u;
+    const bool all_full = __ballot(lane < count && !full) == 0;
+    const std::uint32_t flags = across == qa && down == qd && all_full ? TSK_BLOCK_FULL : 0u;
+
+    auto *out = reinterpret_cast<std::uint32_t *>(a.blocks + b);
+    const auto *image_words = reinterpret_cast<const std::uint32_t *>(img);
+#pragma unroll
+    for (std::uint32_t pass = 0; pass < 2; ++pass) {
+        const std::uint32_t w = lane + 64u * pass;
+        // Tile words: field (w - 8) % 8 of tile (w - 8) / 8, from that tile's lane.
+        const std::uint32_t src = w >= 8u ? ((w - 8u) / 8u) & 7u : 0u, field = (w - 8u) & 7u;
+        std::uint32_t tile_word = 0;
+#pragma unroll
+        for (std::uint32_t k = 0; k < 6; ++k) {
+            const std::uint32_t v = static_cast<std::uint32_t>(__shfl(static_cast<int>(f[k]), static_cast<int>(src)));
+            if (field == k) tile_word = v;
+        }
+        std::uint32_t v = 0;
+        if (w < 8u) {
+            v = w == 0 ? r : w == 1 ? (across | (down << 16)) : w == 2 ? col0 : w == 3 ? row0 : w == 4 ? count
+                : w == 5 ? total : w == 6 ? flags : 0u;
+        } else if (w < 72u) {
+            v = tile_word;
+        } else if (w < 102u) {
+            v = image_words[w - 72u];
+        }
+        if (w < sizeof(tsk_block) / 4) out[w] = v;
+    }
+}
+
 template <int BLOCK> DecodeFn block_function(std::uint32_t kind) {
     switch (kind) {
     case TSK_KIND_U16: return block_decode_kernel<TSK_KIND_U16, BLOCK>;
@@ -446,6 +526,15 @@ template <int BLOCK> DecodeFn block_function(std::uint32_t kind) {
 } // namespace tsk
 
 extern "C" {
+
+int tsk_build_blocks(const tsk_block_build_args *args, void *stream) {
+    if (args->block_count == 0) return hipSuccess;
+    const std::uint64_t grid = (args->block_count + 3) / 4;
+    if (grid > 0x7fffffffu) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(tsk::build_blocks_kernel, dim3(static_cast<std::uint32_t>(grid)), dim3(256), 0,
+                       static_cast<hipStream_t>(stream), *args);
+    return hipGetLastError();
+}
 
 // The block decoder of a kind for 512 threads per workgroup (one row offset
 // entry per thread).

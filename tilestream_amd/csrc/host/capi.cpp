@@ -81,6 +81,7 @@ static_assert(offsetof(ts_server_options, staging_bytes) == offsetof(hip::Server
 static_assert(offsetof(ts_archive_info, host) == offsetof(hip::ArchiveInfo, host));
 static_assert(offsetof(ts_server_options, shape) == offsetof(hip::ServerOptions, shape));
 static_assert(offsetof(ts_server_options, large_bytes) == offsetof(hip::ServerOptions, large_bytes));
+static_assert(offsetof(ts_server_options, block_tasks) == offsetof(hip::ServerOptions, block_tasks));
 static_assert(offsetof(ts_archive_info, tile_count) == offsetof(hip::ArchiveInfo, tile_count));
 
 namespace {

@@ -302,9 +302,11 @@ namespace {
 //
 // Table layout (host and device alike): work counter | DeviceResult[n]
 // (region requests first, then encode requests) | tsk_decode_image[regions] |
-// tsk_request[regions] | request of every decode task | staging block of every
-// decode task | tsk_encode_image[encodes] (uint16 images, then float32) |
-// encode window entries | per window, the entry of each block of 64 tiles.
+// tsk_request[regions] | request of every decode task (batches decoded by
+// blocks: of every block) | staging block of every decode task |
+// tsk_encode_image[encodes] (uint16 images, then float32) | encode window
+// entries | per window, the entry of each block of 64 tiles.  Block records,
+// built on the device, have a device buffer of their own.
 struct Slot {
     hipStream_t stream = nullptr;
     hipEvent_t done = nullptr;
@@ -313,6 +315,7 @@ struct Slot {
     std::uint8_t *device = nullptr;
     std::uint8_t *scratch = nullptr;   // encoder workspace (look-back buffers, carried cursor)
     std::uint8_t *staging = nullptr;   // payloads of tiles of host archives
+    tsk_block *records = nullptr;      // block records of a batch decoded by blocks
     std::uint64_t published = 0;       // offset of the published results in `host`
     std::vector<std::uint64_t> order;  // request index of each result of the current batch
     std::uint64_t bytes = 0;           // output bytes admitted
@@ -321,6 +324,28 @@ struct Slot {
 };
 
 constexpr std::uint64_t kCounterBytes = 256;
+
+// Block records per slot (416 bytes each, 27 MB): a batch that needs more is
+// decoded tile by tile.
+constexpr std::uint64_t kBlockRecords = 65536;
+
+// The block geometry of a region request, as the host block planner makes it
+// (plan_blocks): across x down tiles whose output is at most 64 rows of at
+// most 1 KiB.  False when the block decoder cannot take the request: a host
+// archive (its payloads are staged tile by tile), or a geometry whose blocks
+// could exceed the block stage with every tile RAW and 16-byte misaligned.
+bool block_geometry(const ArchiveInfo &info, bool calibrated, std::uint32_t &across, std::uint32_t &down) {
+    if (info.host) return false;
+    const std::uint32_t stored = info.image.type == PixelType::uint16 ? 2u : 4u;
+    const std::uint32_t out = calibrated ? 4u : stored;
+    const std::uint32_t tw = info.tile.width, th = info.tile.height;
+    if (tw == 0 || th == 0) return false;
+    across = std::max(1u, 1024u / (tw * out));
+    down = std::max(1u, 64u / th);
+    if (across * down > 8) return false;
+    const std::uint64_t per_tile = (std::uint64_t{tw} * th * stored + 3) / 4 * 4 / 16 + 2;
+    return across * down * per_tile <= TSK_BLOCK_STAGE_USED;
+}
 
 struct Layout {
     std::uint64_t results, images, requests, index, staged, encode, entries, blocks, total;
@@ -366,6 +391,7 @@ struct Server::Impl {
     int device = 0;
     int cus = 0;
     int per_cu[5] = {};
+    int per_cu_blocks[5] = {}; // the block decoder's resident workgroups per CU, per kind
     bool pageable = false;
     std::vector<Slot> slots;
     std::vector<Extent> extents; // per request of the current replay
@@ -379,6 +405,7 @@ struct Server::Impl {
             if (s.device != nullptr) static_cast<void>(hipFree(s.device));
             if (s.scratch != nullptr) static_cast<void>(hipFree(s.scratch));
             if (s.staging != nullptr) static_cast<void>(hipFree(s.staging));
+            if (s.records != nullptr) static_cast<void>(hipFree(s.records));
         }
     }
 
@@ -419,7 +446,8 @@ Status Server::create(const ArchiveSet &archives, const ServerOptions &options, 
             hipDeviceGetAttribute(&impl->cus, hipDeviceAttributeMultiprocessorCount, impl->device) != hipSuccess)
             return detail::hip_failure();
         for (std::uint32_t kind = 0; kind < 5; ++kind)
-            if (tsk_decode_occupancy(kind, sh.group, sh.block, sh.depth, &impl->per_cu[kind]) != hipSuccess)
+            if (tsk_decode_occupancy(kind, sh.group, sh.block, sh.depth, &impl->per_cu[kind]) != hipSuccess ||
+                tsk_decode_occupancy(kind, 0, 512, 2, &impl->per_cu_blocks[kind]) != hipSuccess)
                 return detail::hip_failure();
         if (tsk_preload(0) != hipSuccess || tsk_preload(1) != hipSuccess || tsk_preload(3) != hipSuccess)
             return detail::hip_failure();
@@ -448,6 +476,11 @@ Status Server::create(const ArchiveSet &archives, const ServerOptions &options, 
             s.device = static_cast<std::uint8_t *>(d);
             if (hipMalloc(&w, Scratch::total(options.encode_window)) != hipSuccess) return detail::hip_failure();
             s.scratch = static_cast<std::uint8_t *>(w);
+            if (options.block_tasks != 0) {
+                void *b = nullptr;
+                if (hipMalloc(&b, kBlockRecords * sizeof(tsk_block)) != hipSuccess) return detail::hip_failure();
+                s.records = static_cast<tsk_block *>(b);
+            }
             if (set->any_host) {
                 void *g = nullptr;
                 if (hipMalloc(&g, options.staging_bytes) != hipSuccess) return detail::hip_failure();
@@ -560,6 +593,21 @@ Status Server::Impl::launch(Slot &slot, const ServeRequest *requests, const std:
     auto *staged = reinterpret_cast<std::uint32_t *>(h + l.staged);
     bool staging = false;
     for (std::uint64_t k = 0; k < count && !staging; ++k) staging = extents[ids[k]].staged != 0;
+    // A batch of at least block_tasks tiles, all of device archives whose
+    // blocks fit the block stage, is decoded by blocks: fewer, larger units
+    // (up to 8 tiles), each writing whole 1 KiB runs of its output rows.
+    bool by_blocks = slot.records != nullptr && regions != 0 && !staging && tasks >= options.block_tasks;
+    std::uint64_t block_total = 0;
+    for (std::uint64_t k = 0; k < count && by_blocks; ++k) {
+        const ServeRequest &r = requests[ids[k]];
+        if (r.kind != RequestKind::region) continue;
+        const Extent &e = extents[ids[k]];
+        std::uint32_t across = 0, down = 0;
+        by_blocks = block_geometry(set->archives[r.archive].info, r.calibrated != 0, across, down);
+        block_total += host::ceil_div(e.cols, across) * host::ceil_div(e.rows, down);
+    }
+    by_blocks = by_blocks && block_total <= kBlockRecords;
+    std::uint64_t block_next = 0;
     std::uint64_t staged_block = 0;
     auto *images = reinterpret_cast<tsk_decode_image *>(h + l.images);
     auto *reqs = reinterpret_cast<tsk_request *>(h + l.requests);
@@ -619,8 +667,21 @@ Status Server::Impl::launch(Slot &slot, const ServeRequest *requests, const std:
         q.col0 = e.col0;
         q.row0 = e.row0;
         q.cols = e.cols;
-        q.reserved = 0;
-        std::fill_n(index + task, e.tasks, static_cast<std::uint32_t>(j));
+        q.rows = e.rows;
+        q.first_block = static_cast<std::uint32_t>(block_next);
+        q.across = 0;
+        q.down = 0;
+        if (by_blocks) {
+            std::uint32_t across = 0, down = 0;
+            static_cast<void>(block_geometry(a.info, r.calibrated != 0, across, down));
+            q.across = static_cast<std::uint16_t>(across);
+            q.down = static_cast<std::uint16_t>(down);
+            const std::uint64_t nb = host::ceil_div(e.cols, across) * host::ceil_div(e.rows, down);
+            std::fill_n(index + block_next, nb, static_cast<std::uint32_t>(j));
+            block_next += nb;
+        } else {
+            std::fill_n(index + task, e.tasks, static_cast<std::uint32_t>(j));
+        }
         if (staging) {
             // Tiles of a host archive get consecutive 16-byte blocks of staging.
             for (std::uint64_t k = 0; k < e.tasks; ++k) {
@@ -651,33 +712,47 @@ Status Server::Impl::launch(Slot &slot, const ServeRequest *requests, const std:
         if (u16 != f32 && cal != plain)
             kind = u16 ? (cal ? TSK_KIND_U16_CAL : TSK_KIND_U16) : (cal ? TSK_KIND_F32_CAL : TSK_KIND_F32);
         const LaunchShape &sh = options.shape;
-        std::uint64_t resident =
-            static_cast<std::uint64_t>(cus) * static_cast<std::uint64_t>(std::max(1, per_cu[kind]));
+        std::uint64_t resident = static_cast<std::uint64_t>(cus) *
+                                 static_cast<std::uint64_t>(std::max(1, by_blocks ? per_cu_blocks[kind] : per_cu[kind]));
         // A batch of large requests is a persistent grid that holds its CUs to
         // the end: it takes (streams - 1) / streams of them, so batches of small
         // requests on the other streams find CUs free.
         if (large && slots.size() > 1) resident = std::max<std::uint64_t>(1, resident * (slots.size() - 1) / slots.size());
-        const std::uint64_t grid = sh.grid != 0 ? sh.grid : std::min<std::uint64_t>(resident, tasks);
-        const bool queued = sh.chunk != 0 && tasks > grid;
+        const std::uint64_t units = by_blocks ? block_total : tasks;
+        const std::uint64_t grid = sh.grid != 0 && !by_blocks ? sh.grid : std::min<std::uint64_t>(resident, units);
+        const std::uint32_t chunk = by_blocks ? 1u : sh.chunk;
+        const bool queued = chunk != 0 && units > grid;
         tsk_decode_args args{};
         args.images = reinterpret_cast<const tsk_decode_image *>(slot.device + l.images);
         args.tasks = set->device_tiles;
         args.rows = set->device_rows;
         args.results = slot.device + l.results;
         args.queue = queued ? reinterpret_cast<std::uint64_t *>(slot.device) : nullptr;
-        args.task_count = tasks;
+        args.task_count = units;
         args.row_count = set->row_count;
         args.image_count = static_cast<std::uint32_t>(regions);
         args.kind = kind;
-        args.group = sh.group;
-        args.block = sh.block;
+        args.group = by_blocks ? 0u : sh.group;
+        args.block = by_blocks ? 512u : sh.block;
         args.grid = static_cast<std::uint32_t>(std::max<std::uint64_t>(1, grid));
-        args.chunk = queued ? sh.chunk : 0;
-        args.depth = sh.depth;
-        args.run = sh.run;
-        args.request_count = static_cast<std::uint32_t>(regions);
-        args.requests = reinterpret_cast<const tsk_request *>(slot.device + l.requests);
-        args.request_index = reinterpret_cast<const std::uint32_t *>(slot.device + l.index);
+        args.chunk = queued ? chunk : 0;
+        args.depth = by_blocks ? 2u : sh.depth;
+        args.run = by_blocks ? 0u : sh.run;
+        if (by_blocks) {
+            tsk_block_build_args build{};
+            build.requests = reinterpret_cast<const tsk_request *>(slot.device + l.requests);
+            build.block_index = reinterpret_cast<const std::uint32_t *>(slot.device + l.index);
+            build.images = args.images;
+            build.tiles = set->device_tiles;
+            build.blocks = slot.records;
+            build.block_count = block_total;
+            if (tsk_build_blocks(&build, slot.stream) != hipSuccess) return detail::hip_failure();
+            args.blocks = slot.records;
+        } else {
+            args.request_count = static_cast<std::uint32_t>(regions);
+            args.requests = reinterpret_cast<const tsk_request *>(slot.device + l.requests);
+            args.request_index = reinterpret_cast<const std::uint32_t *>(slot.device + l.index);
+        }
         if (staging) {
             args.staged = reinterpret_cast<const std::uint32_t *>(slot.device + l.staged);
             args.staging = slot.staging;
