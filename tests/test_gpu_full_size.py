@@ -7,6 +7,8 @@ are covered in test_gpu_behaviour.py):
   3.65 GB decoded), through both decoders and a captured graph: every image
   equal to the pixels it was encoded from, every result record success with
   the image's byte count;
+* the GPU encoder over the same images (rows on 256 bytes), every archive
+  byte-identical to the host codec's;
 * region serving over 64 resident 4096 x 4096 uint16 archives (64 x 64
   tiles): 10,000 region requests with log-uniform sides in [32, 2048], each
   fused with (x - 100) * 1.05 to FP32, served by the continuous-batching
@@ -55,6 +57,54 @@ def test_full_batch_decodes_every_image(full_batch, shape):
         for i, e in enumerate(batch.entries):
             assert int(written[i]) == e.raw_bytes
             assert np.array_equal(batch.read_output(i), e.pixels), (shape, replay, i, e.width, e.height, e.tile)
+
+
+def test_full_batch_encodes_every_image(full_batch):
+    """The GPU encoder over the same 256 images, rows on 256 bytes (as the
+    decode above writes them), one compress plan per pixel type and tile
+    shape: every archive byte-identical to the host codec's, which equals the
+    oracle on the golden vectors."""
+    batch = full_batch
+    offsets, pitches, total = [], [], 0
+    for e in batch.entries:
+        pitch = (e.width * e.sample + 255) // 256 * 256
+        offsets.append(total)
+        pitches.append(pitch)
+        total += (pitch * (e.height - 1) + e.width * e.sample + 255) // 256 * 256
+    raw = api.DeviceMemory(total)
+    for e, off, pitch in zip(batch.entries, offsets, pitches):
+        rows = np.zeros((e.height, pitch), np.uint8)
+        rows[:, :e.width * e.sample] = e.pixels.view(np.uint8).reshape(e.height, -1)
+        raw.upload(rows.reshape(-1)[:pitch * (e.height - 1) + e.width * e.sample], off)
+    groups = {}
+    for i, e in enumerate(batch.entries):
+        groups.setdefault((int(e.ptype), tuple(e.tile)), []).append(i)
+    stream = api.Stream()
+    for (ptype, tile), members in sorted(groups.items()):
+        items, outs = [], []
+        for i in members:
+            e = batch.entries[i]
+            st, bound = api.compress_bound(api.ImageSpec(ptype, e.width, e.height), api.TileShape(*tile))
+            assert st == api.Status.success
+            out = api.DeviceMemory(bound)
+            it = api.BatchItem()
+            it.input = api.device_view(raw, e.ptype, e.width, e.height, pitches[i], offset=offsets[i],
+                                       capacity=pitches[i] * (e.height - 1) + e.width * e.sample, const=True)
+            it.compressed_output = api.MutableBytes(out.ptr, bound)
+            items.append(it)
+            outs.append(out)
+        st, plan = api.prepare_batch(api.Operation.compress, items, api.TileShape(*tile), stream=stream)
+        assert st == api.Status.success, st
+        ws = api.DeviceMemory(max(256, plan.workspace_bytes()))
+        res = api.DeviceMemory(16 * len(members))
+        assert api.compress_async(plan, ws, res, len(members), stream) == api.Status.success
+        stream.synchronize()
+        status, written = api.read_results(res, len(members))
+        assert (status == 0).all(), (ptype, tile)
+        for i, out, n in zip(members, outs, written):
+            archive = batch.entries[i].archive
+            assert int(n) == len(archive), (i, ptype, tile)
+            assert out.download(len(archive)).tobytes() == archive, (i, ptype, tile)
 
 
 def test_full_region_serving_matches_pixels():
