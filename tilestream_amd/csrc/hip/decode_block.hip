@@ -270,7 +270,7 @@ __device__ inline void store_group(const tsk_decode_image &img, gptr<std::uint8_
 }
 
 // A full block (TSK_BLOCK_FULL) inside the region whose output rows start
-// 16-byte aligned: every lane decodes and stores one group in every row it
+// on words: every lane decodes and stores one group in every row it
 // covers, with no mode, edge or region tests.  S (lanes per tile row) is a
 // constant, so the segment scan has no branches and the unrolled rows are
 // straight-line code whose LDS reads the compiler can overlap.
@@ -307,8 +307,9 @@ __device__ inline void block_rows_full(const tsk_decode_image &img, const BlockJ
 }
 
 // Whether the block takes the full-block path: all its tiles predictive and
-// full size, 64 rows, inside the region both ways, output rows 16-byte
-// aligned.
+// full size, 64 rows, inside the region both ways, output rows on words (a
+// group's store is one multi-word access at word alignment, so a region at
+// any column keeps its inner blocks on this path).
 __device__ inline bool block_is_full(const tsk_decode_image &img, const BlockJob &j) {
     if ((j.flags & TSK_BLOCK_FULL) == 0) return false;
     const std::uint32_t s = img.calibrated || img.type != TSK_UINT16 ? 4u : 2u;
@@ -318,7 +319,7 @@ __device__ inline bool block_is_full(const tsk_decode_image &img, const BlockJob
         y1 > std::uint64_t{img.roi_y} + img.roi_h || y1 - y0 != 64)
         return false;
     const std::uintptr_t first = reinterpret_cast<std::uintptr_t>(img.out) + (x0 - img.roi_x) * s;
-    return first % 16 == 0 && img.out_pitch % 16 == 0;
+    return first % 4 == 0 && img.out_pitch % 4 == 0;
 }
 
 template <bool U16, bool CAL, int BLOCK, bool MAPS>
