@@ -61,32 +61,46 @@ template <bool U16> __global__ void __launch_bounds__(256) calibrate_kernel(tsk_
                                       : nullptr;
         const std::uint32_t width = img.width, groups = width / 4;
         const bool in_words = !U16 || reinterpret_cast<std::uintptr_t>(in) % 4 == 0;
-#pragma unroll 2
-        for (std::uint32_t q = threadIdx.x; q < groups; q += blockDim.x) {
-            const std::uint32_t x = 4 * q;
-            float v[4];
-            if constexpr (U16) {
-                if (in_words) {
-                    const u32x2w w = __builtin_nontemporal_load(reinterpret_cast<gptr<const u32x2w>>(in + 2 * x));
-                    v[0] = static_cast<float>(w.x & 0xffffu);
-                    v[1] = static_cast<float>(w.x >> 16);
-                    v[2] = static_cast<float>(w.y & 0xffffu);
-                    v[3] = static_cast<float>(w.y >> 16);
-                } else {
+        // Each lane loads all its groups of a pass (up to kPass) before it
+        // stores any: a wait for a load also waits for every store issued
+        // before it, so loads placed after stores would wait once per group.
+        constexpr std::uint32_t kPass = 4;
+        for (std::uint32_t q0 = threadIdx.x; q0 < groups; q0 += kPass * blockDim.x) {
+            float v[kPass][4];
+            f32x4w o[kPass], g[kPass];
 #pragma unroll
-                    for (int k = 0; k < 4; ++k) v[k] = sample<true>(in, x + k);
+            for (std::uint32_t k = 0; k < kPass; ++k) {
+                const std::uint32_t q = q0 + k * blockDim.x;
+                if (q >= groups) break;
+                const std::uint32_t x = 4 * q;
+                if constexpr (U16) {
+                    if (in_words) {
+                        const u32x2w w = __builtin_nontemporal_load(reinterpret_cast<gptr<const u32x2w>>(in + 2 * x));
+                        v[k][0] = static_cast<float>(w.x & 0xffffu);
+                        v[k][1] = static_cast<float>(w.x >> 16);
+                        v[k][2] = static_cast<float>(w.y & 0xffffu);
+                        v[k][3] = static_cast<float>(w.y >> 16);
+                    } else {
+#pragma unroll
+                        for (int c = 0; c < 4; ++c) v[k][c] = sample<true>(in, x + c);
+                    }
+                } else {
+                    const f32x4w w = __builtin_nontemporal_load(reinterpret_cast<gptr<const f32x4w>>(in + 4 * x));
+                    v[k][0] = w.x, v[k][1] = w.y, v[k][2] = w.z, v[k][3] = w.w;
                 }
-            } else {
-                const f32x4w w = __builtin_nontemporal_load(reinterpret_cast<gptr<const f32x4w>>(in + 4 * x));
-                v[0] = w.x, v[1] = w.y, v[2] = w.z, v[3] = w.w;
+                o[k] = om ? __builtin_nontemporal_load(reinterpret_cast<gptr<const f32x4w>>(om + x))
+                          : f32x4w{img.offset, img.offset, img.offset, img.offset};
+                g[k] = gm ? __builtin_nontemporal_load(reinterpret_cast<gptr<const f32x4w>>(gm + x))
+                          : f32x4w{img.gain, img.gain, img.gain, img.gain};
             }
-            f32x4w o = {img.offset, img.offset, img.offset, img.offset};
-            f32x4w g = {img.gain, img.gain, img.gain, img.gain};
-            if (om) o = __builtin_nontemporal_load(reinterpret_cast<gptr<const f32x4w>>(om + x));
-            if (gm) g = __builtin_nontemporal_load(reinterpret_cast<gptr<const f32x4w>>(gm + x));
-            const f32x4w r = {calibrate(v[0], o.x, g.x), calibrate(v[1], o.y, g.y), calibrate(v[2], o.z, g.z),
-                              calibrate(v[3], o.w, g.w)};
-            __builtin_nontemporal_store(r, reinterpret_cast<gptr<f32x4w>>(out + x));
+#pragma unroll
+            for (std::uint32_t k = 0; k < kPass; ++k) {
+                const std::uint32_t q = q0 + k * blockDim.x;
+                if (q >= groups) break;
+                const f32x4w r = {calibrate(v[k][0], o[k].x, g[k].x), calibrate(v[k][1], o[k].y, g[k].y),
+                                  calibrate(v[k][2], o[k].z, g[k].z), calibrate(v[k][3], o[k].w, g[k].w)};
+                __builtin_nontemporal_store(r, reinterpret_cast<gptr<f32x4w>>(out + 4 * q));
+            }
         }
         const std::uint32_t x = 4 * groups + threadIdx.x;
         if (x < width) out[x] = calibrate(sample<U16>(in, x), om ? om[x] : img.offset, gm ? gm[x] : img.gain);
