@@ -77,6 +77,25 @@ def test_standalone_matches_host_binary32(orc):
     assert ok, bad
 
 
+@pytest.mark.parametrize("w", [1031, 4099, 9001])
+def test_standalone_wide_rows(orc, w):
+    """Rows of more than 1024 pixels: a lane loads up to four groups of a row
+    (1024 pixels apart) before it stores them, and rows wider than 4096
+    pixels take more than one such pass, then the last w % 4 pixels one per
+    lane.  Odd widths put every other uint16 row off a 4-byte boundary."""
+    h = 3
+    px = g.detector_u16(w, h, 21)
+    dark = g.dark_u16(w, h, 22).astype(np.float32)
+    gain = g.gain_f32(w, h, 23)
+    assert np.array_equal(calibrate_on_gpu(px, dark, gain), orc.calibrate(px, dark, gain).view(np.uint32))
+    assert np.array_equal(calibrate_on_gpu(px, dark, 1.25), orc.calibrate(px, dark, 1.25).view(np.uint32))
+    f = g.field_f32(w, h, 24)
+    ok, bad = same_or_both_nan(calibrate_on_gpu(f, 100.0, gain), orc.calibrate(f, 100.0, gain).view(np.uint32))
+    assert ok, bad
+    ok, bad = same_or_both_nan(calibrate_on_gpu(f, 100.0, 1.05), orc.calibrate(f, 100.0, 1.05).view(np.uint32))
+    assert ok, bad
+
+
 def test_every_uint16_value():
     px = np.arange(65536, dtype=np.uint32).astype(np.uint16).reshape(256, 256)
     got = calibrate_on_gpu(px, 100.25, 1.5).view(np.float32)
